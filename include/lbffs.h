@@ -1,0 +1,156 @@
+/*
+ * lbffs.h -- C ABI of the MI355X lattice-Boltzmann time-step engine.
+ *
+ * The engine replaces the per-step device work of LBFFS: the OpenCL program build,
+ * the device buffers and the k_streamingCollision launch of the reference host
+ * (D3Q19_opencl/ns.cpp:382-603, D3Q27_opencl/ns.cpp same structure).  The host keeps
+ * its own arrays in the reference layout (populations q-major, f[q*N + ic],
+ * ic = i + nx*(j + ny*k)) and hands them across this boundary as plain pointers.
+ *
+ * Conventions
+ *  - every entry point returns int: 0 on success, a negative LBFFS_E* code on error
+ *    (as cl_int does for the OpenCL calls it replaces); lbffs_strerror() names it and
+ *    lbffs_last_error() gives the detail text of the last failure on this thread;
+ *  - "host real" means float, or double when the context stores LBFFS_STORE_F64;
+ *  - calls on one context are not thread-safe; lbffs_download/lbffs_reduce block
+ *    until the device work before them has finished (like cl::copy after
+ *    queue.finish(), ns.cpp:537 / write.hpp:3-15); lbffs_step returns as soon as the
+ *    work is queued.
+ */
+#ifndef LBFFS_H
+#define LBFFS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBFFS_ABI_VERSION 1
+
+/* ---- error codes ---- */
+enum {
+    LBFFS_OK = 0,
+    LBFFS_E_INVALID = -1,     /* bad argument or size */
+    LBFFS_E_NOMEM = -2,       /* device allocation failed */
+    LBFFS_E_DEVICE = -3,      /* HIP runtime error */
+    LBFFS_E_UNSUPPORTED = -4, /* configuration this build does not implement */
+    LBFFS_E_STATE = -5        /* call out of order (e.g. step before upload of f) */
+};
+
+/* ---- storage forms of the populations in HBM ---- */
+enum {
+    LBFFS_STORE_F32 = 0,     /* reference form: float f_q */
+    LBFFS_STORE_F16DEV = 1,  /* IEEE half of (f_q - w_q) * 2^e_q, arithmetic in FP32 */
+    LBFFS_STORE_BF16DEV = 2, /* bfloat16 of (f_q - w_q) * 2^e_q, arithmetic in FP32 */
+    LBFFS_STORE_F64 = 3      /* double storage and double arithmetic (accuracy reference) */
+};
+
+/* ---- sub-grid-scale models ---- */
+enum {
+    LBFFS_SGS_WALE = 0,        /* waleTauSGS, the model the reference kernel runs (K19:3631) */
+    LBFFS_SGS_SMAGORINSKY = 1, /* smagorinskyTauSGS with Fluent-style wall damping (K19:2133-2201) */
+    LBFFS_SGS_NONE = 2         /* tauSGS = 0 */
+};
+
+/* ---- collision operators ---- */
+enum {
+    LBFFS_COLL_RR = 0,      /* recursive regularized, K19:3156-3309 (D3Q19) */
+    LBFFS_COLL_CUMULANT = 1 /* parametrized cumulant, D3Q27_opencl/kernel.cl:2352-2691 (D3Q27) */
+};
+
+/* ---- fields exchanged with the host (names of the reference buffers, ns.cpp:405-436) ---- */
+enum {
+    LBFFS_FIELD_F = 0,         /* Q*N host reals, q-major */
+    LBFFS_FIELD_BOUNDARY = 1,  /* N int32: 1 on domain faces (ns.cpp:164-177) */
+    LBFFS_FIELD_BOUNDARY1 = 2, /* N int32: x-face BC code (input.hpp:221-256) */
+    LBFFS_FIELD_BOUNDARY2 = 3, /* N int32: y-face BC code */
+    LBFFS_FIELD_BOUNDARY3 = 4, /* N int32: z-face BC code */
+    LBFFS_FIELD_SDF = 5,       /* N host reals */
+    LBFFS_FIELD_SOLID = 6,     /* N uint8 */
+    LBFFS_FIELD_NEISOLID = 7,  /* N uint8 */
+    LBFFS_FIELD_U0 = 8,        /* N host reals, lattice units */
+    LBFFS_FIELD_V0 = 9,
+    LBFFS_FIELD_W0 = 10,
+    LBFFS_FIELD_FWX = 11, /* N host reals: momentum-exchange force on solid links */
+    LBFFS_FIELD_FWY = 12,
+    LBFFS_FIELD_FWZ = 13,
+    LBFFS_FIELD_TAUSGS = 14, /* N host reals */
+    LBFFS_FIELD_RHO = 15,    /* N host reals: pre-collision moments of the last step */
+    LBFFS_FIELD_U = 16,
+    LBFFS_FIELD_V = 17,
+    LBFFS_FIELD_W = 18,
+    LBFFS_FIELD_OMEGA = 19, /* N host reals: 1/tau per cell */
+    LBFFS_FIELD_GX = 20,    /* N host reals: IBM forcing density */
+    LBFFS_FIELD_GY = 21,
+    LBFFS_FIELD_GZ = 22,
+    LBFFS_FIELD_COUNT = 23
+};
+
+/* Fixed per-run description of a lattice.  Replaces the program build and buffer
+ * creation of ns.cpp:385-436. */
+typedef struct {
+    int abi_version; /* LBFFS_ABI_VERSION */
+    int q;           /* 19 (27 reserved) */
+    int nx, ny, nz;
+    int storage;   /* LBFFS_STORE_* */
+    int sgs;       /* LBFFS_SGS_* */
+    int collision; /* LBFFS_COLL_* */
+    float omega0;  /* 1/(3 nu + 1/2): cells whose omega equals it skip the omega read */
+    int block;     /* work-group size; 0 = engine default */
+    int reserved[8];
+} lbffs_desc;
+
+/* Scalars of one k_streamingCollision launch (ns.cpp:528-535, kernel.cl:3541-3545). */
+typedef struct {
+    float dpdx;      /* body force along x, lattice units */
+    float LES;       /* 1 or 0: multiplies the Smagorinsky tauSGS (WALE ignores it) */
+    float omegaB;    /* bulk relaxation rate of the RR collision */
+    float spzWidth;  /* sponge width as a fraction of the domain (tauSpongeZone) */
+    float Cs;        /* Smagorinsky constant */
+    int damping;     /* Smagorinsky wall damping on (1) / off (0) */
+    int store_every; /* 1: write rho/u/v/w/tauSGS/Fw/G for every cell on every step
+                        (reference semantics throughout); 0: only on the last step of
+                        the call and for the cells later steps read */
+    int reserved[8];
+} lbffs_step_params;
+
+typedef struct lbffs_ctx lbffs_ctx;
+
+/* Create a context on the current HIP device (the OpenCL context + buffers of
+ * ns.cpp:385-436).  Every field is zero-initialised except f, which must be uploaded. */
+int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out);
+
+/* Copy a host array into a device field (the cl::Buffer fills of ns.cpp:405-436).
+ * bytes must equal the field's host size. */
+int lbffs_upload(lbffs_ctx *ctx, int field, const void *host, size_t bytes);
+
+/* Advance nsteps time steps (the loop body ns.cpp:514-537 + swap ns.cpp:602). */
+int lbffs_step(lbffs_ctx *ctx, int nsteps, const lbffs_step_params *params);
+
+/* Copy a device field to the host (cl::copy of write.hpp:3-15).  F is returned in
+ * the host real type whatever the storage form. */
+int lbffs_download(lbffs_ctx *ctx, int field, void *host, size_t bytes);
+
+/* FP64 sums over fluid cells (the host loops of write.hpp:16-47, on the device):
+ * out[0..2] = sum Fw{x,y,z}; out[3..5] = sum G{x,y,z}; out[6] = sum (rho - 1) with rho
+ * summed from the current populations; out[7] = number of fluid cells. */
+#define LBFFS_REDUCE_N 8
+int lbffs_reduce(lbffs_ctx *ctx, double out[LBFFS_REDUCE_N]);
+
+/* Wait for queued device work. */
+int lbffs_synchronize(lbffs_ctx *ctx);
+
+/* Device time of the last lbffs_step call in milliseconds (HIP events around it). */
+int lbffs_last_step_ms(lbffs_ctx *ctx, float *ms);
+
+int lbffs_destroy(lbffs_ctx *ctx);
+
+const char *lbffs_strerror(int code);
+const char *lbffs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,68 @@
+/*
+ * lbffs_case.h -- host-side case setup of the LBFFS driver, as a C ABI.
+ *
+ * These functions build, on the CPU, the per-cell arrays that the reference host
+ * prepares before its time loop and that the step kernel consumes (lbffs.h fields):
+ * unit conversion, face BC codes and wall velocities, the domain-face flag, the
+ * initial equilibrium populations, the solid-neighbour flag and the outlet-patch
+ * omega sponge.  Results are bit-identical to the reference's host code for the same
+ * inputs (x86-64, no FMA contraction).
+ */
+#ifndef LBFFS_CASE_H
+#define LBFFS_CASE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Faces in the order of boundaryConditions.txt: nxMin, nxMax, nyMin, nyMax, nzMin, nzMax. */
+enum { LBFFS_NXMIN = 0, LBFFS_NXMAX = 1, LBFFS_NYMIN = 2, LBFFS_NYMAX = 3, LBFFS_NZMIN = 4, LBFFS_NZMAX = 5 };
+
+/* The input.txt quantities the step depends on (input.hpp:115-219), physical units. */
+typedef struct {
+    int nx, ny, nz;
+    float Lx, uMax, rho0, U0, nu, dpdx;
+    float uIni, vIni, wIni;
+    int face_code[6][3];  /* <face>Boundary1..3 as BC codes (input.hpp:221-256) */
+    float face_vel[6][3]; /* <face>U0,V0,W0 in m/s */
+} lbffs_case_in;
+
+/* Derived lattice quantities (ns.cpp:104-128, input.hpp:282-287). */
+typedef struct {
+    float L;        /* cell size Lx/(nx-1) */
+    float c;        /* velocity scale uMax/U0 */
+    float Re;       /* uMax*Lx/nu */
+    float nu;       /* lattice viscosity */
+    float dpdx;     /* lattice body force */
+    float uIni, vIni, wIni;
+    float omega0;   /* 1/(3 nu + 1/2) */
+    float deltaT;   /* L/c */
+    float face_vel[6][3]; /* face velocities in lattice units (divided by c) */
+} lbffs_units;
+
+/* ns.cpp:104-128 and the /c of input.hpp:282-287. */
+int lbffs_case_units(const lbffs_case_in *in, lbffs_units *out);
+
+/* setBoundaryConditions.hpp:1-186 and the face flag loop ns.cpp:164-177.
+ * Arrays of nx*ny*nz entries; they are fully overwritten. */
+int lbffs_case_faces(const lbffs_case_in *in, const lbffs_units *un, int32_t *boundary, int32_t *b1, int32_t *b2,
+                     int32_t *b3, float *u0, float *v0, float *w0);
+
+/* initialization.hpp:1-27: f = fTmp = (rho + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2) w_q, q-major. */
+int lbffs_case_init_feq(int q, int nx, int ny, int nz, float rho0, float u, float v, float w, float *f);
+
+/* walls.hpp:187-216 setNeiSolid (upwind neighbours with periodic wrap, faces excluded). */
+int lbffs_case_neisolid(int q, int nx, int ny, int nz, const uint8_t *solid, uint8_t *neiSolid);
+
+/* ns.cpp:343-375: cosine omega sponge of width spz cells around one outlet-patch cell
+ * (i,j,k) with face codes (p1,p2,p3); omega is updated in place. */
+int lbffs_case_outlet_sponge(int nx, int ny, int nz, int i, int j, int k, int p1, int p2, int p3, int spz,
+                             float omega0, float *omega);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

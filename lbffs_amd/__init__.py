@@ -1,0 +1,176 @@
+"""lbffs_amd -- MI355X lattice-Boltzmann time-step engine for the LBFFS solver.
+
+Python view of the C ABI in ``include/lbffs.h`` (device step) and
+``include/lbffs_case.h`` (host case setup).  The library ``lbffs_amd/liblbffs.so`` is
+built by ``python -m lbffs_amd.build``; importing this package never builds it and
+never falls back to another implementation: without the library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblbffs.so")
+
+ABI_VERSION = 1
+
+# storage forms, SGS models, collisions (lbffs.h)
+STORE_F32, STORE_F16DEV, STORE_BF16DEV, STORE_F64 = 0, 1, 2, 3
+SGS_WALE, SGS_SMAGORINSKY, SGS_NONE = 0, 1, 2
+COLL_RR, COLL_CUMULANT = 0, 1
+
+# BC codes (input.hpp:221-256)
+BC = {"Cyclic": 0, "BounceBack": 1, "Outlet": 3, "Symmetry": 4, "FixedVelocity": 5,
+      "FixedDensity": 6, "WallFunction": 7}
+
+FIELDS = ["f", "boundary", "boundary1", "boundary2", "boundary3", "sdf", "solid", "neiSolid",
+          "u0", "v0", "w0", "Fwx", "Fwy", "Fwz", "tauSGS", "rho", "u", "v", "w", "omega",
+          "GxIBM", "GyIBM", "GzIBM"]
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+INT_FIELDS = {"boundary", "boundary1", "boundary2", "boundary3"}
+U8_FIELDS = {"solid", "neiSolid"}
+REDUCE_N = 8
+
+
+class LbffsError(RuntimeError):
+    pass
+
+
+class Desc(C.Structure):
+    _fields_ = [("abi_version", C.c_int), ("q", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("storage", C.c_int), ("sgs", C.c_int), ("collision", C.c_int), ("omega0", C.c_float),
+                ("block", C.c_int), ("reserved", C.c_int * 8)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [("dpdx", C.c_float), ("LES", C.c_float), ("omegaB", C.c_float), ("spzWidth", C.c_float),
+                ("Cs", C.c_float), ("damping", C.c_int), ("store_every", C.c_int), ("reserved", C.c_int * 8)]
+
+
+class CaseIn(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("Lx", C.c_float), ("uMax", C.c_float),
+                ("rho0", C.c_float), ("U0", C.c_float), ("nu", C.c_float), ("dpdx", C.c_float),
+                ("uIni", C.c_float), ("vIni", C.c_float), ("wIni", C.c_float),
+                ("face_code", (C.c_int * 3) * 6), ("face_vel", (C.c_float * 3) * 6)]
+
+
+class Units(C.Structure):
+    _fields_ = [("L", C.c_float), ("c", C.c_float), ("Re", C.c_float), ("nu", C.c_float), ("dpdx", C.c_float),
+                ("uIni", C.c_float), ("vIni", C.c_float), ("wIni", C.c_float), ("omega0", C.c_float),
+                ("deltaT", C.c_float), ("face_vel", (C.c_float * 3) * 6)]
+
+
+_lib = None
+
+
+def lib():
+    """Load liblbffs.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LbffsError(f"{LIB_PATH} not found: run `python -m lbffs_amd.build` first")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, sz = C.c_void_p, C.c_int, C.c_size_t
+        L.lbffs_create.argtypes = [C.POINTER(Desc), C.POINTER(vp)]
+        L.lbffs_upload.argtypes = [vp, ip, vp, sz]
+        L.lbffs_download.argtypes = [vp, ip, vp, sz]
+        L.lbffs_step.argtypes = [vp, ip, C.POINTER(StepParams)]
+        L.lbffs_reduce.argtypes = [vp, C.POINTER(C.c_double)]
+        L.lbffs_synchronize.argtypes = [vp]
+        L.lbffs_last_step_ms.argtypes = [vp, C.POINTER(C.c_float)]
+        L.lbffs_destroy.argtypes = [vp]
+        L.lbffs_strerror.argtypes = [ip]
+        L.lbffs_strerror.restype = C.c_char_p
+        L.lbffs_last_error.restype = C.c_char_p
+        L.lbffs_case_units.argtypes = [C.POINTER(CaseIn), C.POINTER(Units)]
+        L.lbffs_case_faces.argtypes = [C.POINTER(CaseIn), C.POINTER(Units)] + [vp] * 7
+        L.lbffs_case_init_feq.argtypes = [ip, ip, ip, ip, C.c_float, C.c_float, C.c_float, C.c_float, vp]
+        L.lbffs_case_neisolid.argtypes = [ip, ip, ip, ip, vp, vp]
+        L.lbffs_case_outlet_sponge.argtypes = [ip] * 10 + [C.c_float, vp]
+        for n in ("lbffs_create", "lbffs_upload", "lbffs_download", "lbffs_step", "lbffs_reduce",
+                  "lbffs_synchronize", "lbffs_last_step_ms", "lbffs_destroy", "lbffs_case_units",
+                  "lbffs_case_faces", "lbffs_case_init_feq", "lbffs_case_neisolid", "lbffs_case_outlet_sponge"):
+            getattr(L, n).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        L = lib()
+        raise LbffsError(f"{what}: {L.lbffs_strerror(rc).decode()} ({rc}): {L.lbffs_last_error().decode()}")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Lattice:
+    """One device context (lbffs_create .. lbffs_destroy)."""
+
+    def __init__(self, nx, ny, nz, q=19, storage=STORE_F32, sgs=SGS_WALE, collision=COLL_RR, omega0=1.0,
+                 block=0):
+        self.nx, self.ny, self.nz, self.q = nx, ny, nz, q
+        self.N = nx * ny * nz
+        self.storage = storage
+        self.real = np.float64 if storage == STORE_F64 else np.float32
+        d = Desc(abi_version=ABI_VERSION, q=q, nx=nx, ny=ny, nz=nz, storage=storage, sgs=sgs,
+                 collision=collision, omega0=omega0, block=block)
+        h = C.c_void_p()
+        _check(lib().lbffs_create(C.byref(d), C.byref(h)), "lbffs_create")
+        self._h = h
+
+    def _dtype(self, name):
+        if name in INT_FIELDS:
+            return np.int32
+        if name in U8_FIELDS:
+            return np.uint8
+        return self.real
+
+    def upload(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=self._dtype(name))
+        _check(lib().lbffs_upload(self._h, FIELD_ID[name], _ptr(a), a.nbytes), f"upload {name}")
+
+    def download(self, name):
+        n = self.q * self.N if name == "f" else self.N
+        a = np.empty(n, dtype=self._dtype(name))
+        _check(lib().lbffs_download(self._h, FIELD_ID[name], _ptr(a), a.nbytes), f"download {name}")
+        return a
+
+    def step(self, nsteps, dpdx=0.0, LES=0.0, omegaB=0.1, spzWidth=0.1, Cs=0.1, damping=1, store_every=0):
+        p = StepParams(dpdx=dpdx, LES=LES, omegaB=omegaB, spzWidth=spzWidth, Cs=Cs, damping=damping,
+                       store_every=store_every)
+        _check(lib().lbffs_step(self._h, int(nsteps), C.byref(p)), "lbffs_step")
+
+    def reduce(self):
+        out = (C.c_double * REDUCE_N)()
+        _check(lib().lbffs_reduce(self._h, out), "lbffs_reduce")
+        return np.array(out[:], dtype=np.float64)
+
+    def synchronize(self):
+        _check(lib().lbffs_synchronize(self._h), "lbffs_synchronize")
+
+    def last_step_ms(self):
+        ms = C.c_float()
+        _check(lib().lbffs_last_step_ms(self._h, C.byref(ms)), "lbffs_last_step_ms")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lbffs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,58 @@
+"""Build recipe of the engine: compiles the HIP library for gfx950 in-tree.
+
+    python -m lbffs_amd.build          # product library lbffs_amd/liblbffs.so
+
+The product is one shared library, ``lbffs_amd/liblbffs.so``, holding the C ABI of
+``include/lbffs.h`` (device step, HIP kernels) and ``include/lbffs_case.h`` (host case
+setup).  Host case code is compiled by g++ with FMA contraction off so that its float
+results match the reference host; device code is compiled by hipcc for gfx950 only.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "lbffs_amd")
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(ROOT, "_build")
+LIB = os.path.join(PKG, "liblbffs.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# -packed-fp32-ops: keep the compiler from forming v_pk_*_f32 (build notes)
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-Wno-unused-value",
+]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    if verbose and out.strip():
+        print(out)
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    case_o = os.path.join(BUILD, "lbffs_case.o")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
+          "-c", os.path.join(CSRC, "lbffs_case.cpp"), "-o", case_o], verbose)
+    tmp = LIB + ".tmp"
+    api_o = os.path.join(BUILD, "lbffs_api.o")
+    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "lbffs_api.hip"), "-o", api_o], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", api_o, case_o, "-o", tmp], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-q" not in sys.argv)

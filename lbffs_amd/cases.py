@@ -1,0 +1,199 @@
+"""Case builders: the per-cell arrays of the reference host for the workloads this
+engine is measured and tested on.
+
+Every builder goes through the host case setup of ``include/lbffs_case.h`` (units as
+ns.cpp:104-128, setBoundaryConditions.hpp, the face flag of ns.cpp:164-177,
+initialization.hpp, setNeiSolid) and then adds what the workload defines on top
+(solid geometry, inlet noise, wall distance).  A ``Case`` holds the arrays in the
+reference layout, ready for ``Lattice.upload`` or for the CPU oracle.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import BC, CaseIn, Units, lib, _check, _ptr, SGS_WALE, SGS_SMAGORINSKY, SGS_NONE
+
+FACES = ["nxMin", "nxMax", "nyMin", "nyMax", "nzMin", "nzMax"]
+
+SEED = 20251202  # seed of the stochastic inlet profiles
+
+
+@dataclass
+class Case:
+    name: str
+    nx: int
+    ny: int
+    nz: int
+    arrays: dict
+    omega0: float
+    sgs: int = SGS_WALE
+    params: dict = field(default_factory=dict)  # lbffs_step_params scalars
+    units: dict = field(default_factory=dict)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def N(self):
+        return self.nx * self.ny * self.nz
+
+    def upload_to(self, lat):
+        for k, v in self.arrays.items():
+            lat.upload(k, v)
+
+
+def units(nx, ny, nz, Lx, uMax, U0, nu, rho0=1.0, dpdx=0.0, ini=(0.0, 0.0, 0.0), face_codes=None, face_vels=None):
+    cin = CaseIn(nx=nx, ny=ny, nz=nz, Lx=Lx, uMax=uMax, rho0=rho0, U0=U0, nu=nu, dpdx=dpdx,
+                 uIni=ini[0], vIni=ini[1], wIni=ini[2])
+    for f, name in enumerate(FACES):
+        codes = (face_codes or {}).get(name, ("Cyclic", "Cyclic", "Cyclic"))
+        vel = (face_vels or {}).get(name, (0.0, 0.0, 0.0))
+        for d in range(3):
+            cin.face_code[f][d] = BC[codes[d]] if isinstance(codes[d], str) else int(codes[d])
+            cin.face_vel[f][d] = vel[d]
+    un = Units()
+    _check(lib().lbffs_case_units(C.byref(cin), C.byref(un)), "lbffs_case_units")
+    return cin, un
+
+
+def build_faces(cin, un):
+    N = cin.nx * cin.ny * cin.nz
+    b = [np.zeros(N, np.int32) for _ in range(4)]
+    v = [np.zeros(N, np.float32) for _ in range(3)]
+    _check(lib().lbffs_case_faces(C.byref(cin), C.byref(un), *[_ptr(a) for a in b + v]), "lbffs_case_faces")
+    return b, v
+
+
+def init_feq(q, nx, ny, nz, rho, u, v, w):
+    f = np.empty(q * nx * ny * nz, np.float32)
+    _check(lib().lbffs_case_init_feq(q, nx, ny, nz, rho, u, v, w, _ptr(f)), "lbffs_case_init_feq")
+    return f
+
+
+def neisolid(q, nx, ny, nz, solid):
+    s = np.ascontiguousarray(solid, np.uint8)
+    n = np.zeros_like(s)
+    _check(lib().lbffs_case_neisolid(q, nx, ny, nz, _ptr(s), _ptr(n)), "lbffs_case_neisolid")
+    return n
+
+
+def make_case(name, nx, ny, nz, Lx, uMax, U0, nu, face_codes, face_vels=None, rho0=1.0, dpdx=0.0,
+              ini=(0.0, 0.0, 0.0), solid=None, sgs=SGS_WALE, q=19, **params):
+    cin, un = units(nx, ny, nz, Lx, uMax, U0, nu, rho0, dpdx, ini, face_codes, face_vels)
+    (bnd, b1, b2, b3), (u0, v0, w0) = build_faces(cin, un)
+    N = nx * ny * nz
+    f = init_feq(q, nx, ny, nz, rho0, un.uIni, un.vIni, un.wIni)
+    sol = np.zeros(N, np.uint8) if solid is None else np.ascontiguousarray(solid, np.uint8).reshape(-1)
+    arrays = dict(f=f, boundary=bnd, boundary1=b1, boundary2=b2, boundary3=b3,
+                  sdf=np.full(N, 10000.0, np.float32), solid=sol, neiSolid=neisolid(q, nx, ny, nz, sol),
+                  u0=u0, v0=v0, w0=w0, omega=np.full(N, un.omega0, np.float32))
+    p = dict(dpdx=float(un.dpdx), LES=0.0, omegaB=0.1, spzWidth=0.1, Cs=0.1, damping=1)
+    p.update(params)
+    uu = {k: getattr(un, k) for k in ("L", "c", "Re", "nu", "dpdx", "omega0", "deltaT")}
+    return Case(name, nx, ny, nz, arrays, float(un.omega0), sgs, p, uu)
+
+
+def cavity(n=64, nz=None, u_lid=0.1, Re=100.0, sgs=SGS_WALE):
+    """Lid-driven cavity: lid = nyMax FixedVelocity along +x, other faces half-way
+    bounce-back (testCases/cavityFlow/boundaryConditions.txt:23-28).  Physical inputs are
+    chosen so that the lattice lid speed is u_lid and nu = u_lid*n/Re (Lx=1, uMax=1)."""
+    nz = n if nz is None else nz
+    U0 = u_lid  # uMax/c = U0 -> lid 1 m/s maps to u_lid
+    nu_lat = u_lid * n / Re
+    nu = nu_lat * (1.0 / (n - 1)) * (1.0 / U0)  # inverse of nu/(L c), ns.cpp:111
+    codes = {f: ("BounceBack", "BounceBack", "BounceBack") for f in FACES}
+    codes["nyMax"] = ("BounceBack", "FixedVelocity", "BounceBack")
+    if nz == 1:
+        codes["nzMin"] = codes["nzMax"] = ("Cyclic", "Cyclic", "Cyclic")
+    vels = {"nyMax": (1.0, 0.0, 0.0)}
+    return make_case(f"cavity{n}", n, n, nz, 1.0, 1.0, U0, nu, codes, vels, sgs=sgs)
+
+
+def periodic_box(nx, ny, nz, tau=0.8, ini=(0.0, 0.0, 0.0), sgs=SGS_NONE):
+    """All faces Cyclic: a fully periodic lattice (faces still take the face path)."""
+    nu_lat = (tau - 0.5) / 3.0
+    U0 = 0.1
+    nu = nu_lat * (1.0 / (nx - 1)) * (1.0 / U0)
+    # lattice velocity = physical / c with c = uMax/U0 (ns.cpp:105, 113-115)
+    return make_case(f"box{nx}x{ny}x{nz}", nx, ny, nz, 1.0, 1.0, U0, nu, {}, ini=tuple(x / U0 for x in ini),
+                     sgs=sgs)
+
+
+def poiseuille(nx=100, ny=20, nz=1, dpdx=0.08, nu=0.01, Lx=5.0, U0=0.05, sgs=SGS_NONE):
+    """testCases/poiseuilleFlow: periodic x, half-way bounce-back walls in y, body force."""
+    codes = {f: ("Cyclic", "Cyclic", "Cyclic") for f in FACES}
+    codes["nyMin"] = ("Cyclic", "BounceBack", "Cyclic")
+    codes["nyMax"] = ("Cyclic", "BounceBack", "Cyclic")
+    return make_case("poiseuille", nx, ny, nz, Lx, 1.0, U0, nu, codes, dpdx=dpdx, sgs=sgs)
+
+
+def cylinder_channel(nx=400, ny=160, nz=16, D=40, cx=100, cy=80, u_in=0.05, noise=1e-3, Re=100.0,
+                     spz_cells=20, sgs=SGS_WALE, seed=SEED):
+    """Channel flow past a circular cylinder: FixedVelocity inlet (u_in plus seeded uniform
+    noise per inlet cell), Outlet (Geier convective) at nxMax with a kernel sponge of
+    spz_cells, bounce-back walls in y, periodic z; cylinder cells (i-cx)^2+(j-cy)^2 < (D/2)^2
+    are solid."""
+    U0 = u_in
+    nu_lat = u_in * D / Re
+    nu = nu_lat * (1.0 / (nx - 1)) * (1.0 / U0)
+    codes = {f: ("Cyclic", "Cyclic", "Cyclic") for f in FACES}
+    codes["nxMin"] = ("FixedVelocity", "Cyclic", "Cyclic")
+    codes["nxMax"] = ("Outlet", "Cyclic", "Cyclic")
+    codes["nyMin"] = ("Cyclic", "BounceBack", "Cyclic")
+    codes["nyMax"] = ("Cyclic", "BounceBack", "Cyclic")
+    vels = {"nxMin": (1.0, 0.0, 0.0)}
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    solid = (((i - cx) ** 2 + (j - cy) ** 2) < (D / 2.0) ** 2).astype(np.uint8).reshape(-1)
+    c = make_case("cylinder", nx, ny, nz, 1.0, 1.0, U0, nu, codes, vels, solid=solid, sgs=sgs,
+                  spzWidth=float(spz_cells) / nx)
+    rng = np.random.default_rng(seed)
+    inlet = (i.reshape(-1) == 0) & (c.arrays["boundary1"] == BC["FixedVelocity"])
+    u0 = c.arrays["u0"]
+    u0[inlet] = (u0[inlet].astype(np.float64)
+                 + rng.uniform(-noise, noise, size=int(inlet.sum()))).astype(np.float32)
+    c.meta.update(D=D, cx=cx, cy=cy, u_in=u_in)
+    return c
+
+
+def backward_step(nx=1536, ny=192, nz=128, H=64, step_len=None, u_in=0.05, turb=0.05, Re=5500.0,
+                  spz_cells=64, sgs=SGS_SMAGORINSKY, Cs=0.1, seed=SEED):
+    """Backward-facing step: solid block i < step_len, j < H spanning z; FixedVelocity
+    inlet above the step with seeded 5 % Gaussian turbulence on u0; Outlet + sponge at
+    nxMax; bounce-back y walls; periodic z; Smagorinsky LES with wall damping (wall
+    distance from the step and the y walls).  Solid links of face cells are handled by
+    the reference rule (setNeiSolid leaves faces out)."""
+    step_len = 3 * H if step_len is None else step_len
+    U0 = u_in
+    nu_lat = u_in * H / Re
+    nu = nu_lat * (1.0 / (nx - 1)) * (1.0 / U0)
+    codes = {f: ("Cyclic", "Cyclic", "Cyclic") for f in FACES}
+    codes["nxMin"] = ("FixedVelocity", "Cyclic", "Cyclic")
+    codes["nxMax"] = ("Outlet", "Cyclic", "Cyclic")
+    codes["nyMin"] = ("Cyclic", "BounceBack", "Cyclic")
+    codes["nyMax"] = ("Cyclic", "BounceBack", "Cyclic")
+    vels = {"nxMin": (1.0, 0.0, 0.0)}
+    N = nx * ny * nz
+    ic = np.arange(N, dtype=np.int64)
+    i = ic % nx
+    j = (ic // nx) % ny
+    solid = ((i < step_len) & (j < H)).astype(np.uint8)
+    c = make_case("backstep", nx, ny, nz, 1.0, 1.0, U0, nu, codes, vels, solid=solid, sgs=sgs, LES=1.0, Cs=Cs,
+                  damping=1, spzWidth=float(spz_cells) / nx)
+    rng = np.random.default_rng(seed)
+    inlet = (i == 0) & (c.arrays["boundary1"] == BC["FixedVelocity"]) & (solid == 0)
+    u0 = c.arrays["u0"]
+    u0[inlet] = (u0[inlet].astype(np.float64) * (1.0 + turb * rng.standard_normal(int(inlet.sum())))).astype(np.float32)
+    # wall distance (half-way walls) for cells within 3 cells of the y walls or the step
+    yb = j.astype(np.float64) + 0.5                     # to the nyMin wall
+    yt = (ny - 1 - j).astype(np.float64) + 0.5          # to the nyMax wall
+    ys = np.where(i >= step_len, np.hypot(np.maximum(step_len - 0.5 - i, 0), np.maximum(j - (H - 0.5), 0)),
+                  (j - (H - 0.5)).astype(np.float64))   # to the step surfaces
+    ys = np.where((i >= step_len) & (j >= H), np.hypot(i - (step_len - 0.5), j - (H - 0.5)), ys)
+    ys = np.where((i >= step_len) & (j < H), (i - (step_len - 0.5)).astype(np.float64), ys)
+    d = np.minimum(np.minimum(yb, yt), np.abs(ys))
+    sdf = c.arrays["sdf"]
+    near = (d < 3.0) & (solid == 0)
+    sdf[near] = d[near].astype(np.float32)
+    c.meta.update(H=H, step_len=step_len, u_in=u_in)
+    return c

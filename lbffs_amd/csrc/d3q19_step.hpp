@@ -1,0 +1,671 @@
+// d3q19_step.hpp -- the fused D3Q19 time-step kernel.
+//
+// One work-item per cell, cells flattened as ic = i + nx*(j + ny*k) so that a
+// wavefront's 64 lanes touch 64 consecutive cells of one population array: every
+// pull load and every store is a coalesced run of 256 B (FP32) or 128 B (16-bit).
+// The kernel does, for each fluid cell, what k_streamingCollision does
+// (D3Q19_opencl/kernel.cl:3547-3645):
+//   pull streaming (interior: fixed offsets; faces: wrap or wall per face code),
+//   fixed-velocity / fixed-density / wall-function / outlet face conditions,
+//   simple bounce-back on solid links with momentum-exchange force,
+//   moments, sub-grid model (WALE or Smagorinsky), sponge, recursive-regularized
+//   collision with the dpdx body force.
+// Deliberate differences from the reference's data flow (results are unchanged):
+//   * neighbour reads of rho/u/v/w see the previous step (Jacobi) instead of a
+//     scheduling-dependent mix, via two macro buffers that alternate each step;
+//   * 19 per-cell side arrays are replaced by one 16-bit flag word; side arrays are
+//     read only by the cells whose flag asks for them;
+//   * rho/u/v/w/tauSGS/Fw/G are written for every cell only when the caller asks
+//     (last step of a call or store_every), otherwise only where later steps read them.
+#pragma once
+
+#include "lbffs_common.hpp"
+
+namespace lbffs {
+
+// D3Q19 directions (kernel.cl:3558-3560) as compile-time tables, so that unrolled
+// loops fold the lattice constants into the instruction stream
+struct L19 {
+    static constexpr int cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+    static constexpr int cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+    static constexpr int cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+    __host__ __device__ static constexpr int opp(int q) { return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1); }
+};
+
+template <typename Real>
+__host__ __device__ constexpr Real w19(int q)
+{
+    return q == 0 ? Real(1) / Real(3) : (q <= 6 ? Real(1) / Real(18) : Real(1) / Real(36));
+}
+
+template <typename Real>
+struct Step19Args {
+    uint32_t N;
+    int nx, ny, nz;
+    FastDiv dnx, dny;
+    int off[19];  // interior pull offset: source = ic - off[q]
+    const void *fsrc;
+    void *fdst;
+    const uint16_t *flag;
+    const Real *u0, *v0, *w0, *sdf, *omega;
+    const Real *rho_o, *u_o, *v_o, *w_o;  // macros of the previous step
+    Real *rho_n, *u_n, *v_n, *w_n;        // macros written by this step
+    Real *tau, *Fwx, *Fwy, *Fwz, *Gx, *Gy, *Gz;
+    Real dpdx, LES, omegaB, spz, Cs, omega0;
+    int damping;
+    int mac_all;      // write rho/u/v/w for every fluid cell
+    int aux_all;      // write tauSGS / Fw / G for every fluid cell
+    int sponge_mask;  // faces whose centre cell is Outlet or FixedDensity (tauSpongeZone)
+};
+
+// XCD-aware block order: the hardware deals work-groups round-robin over the 8 XCDs;
+// renumber them so that each XCD walks one contiguous slab of cells and the partial
+// cache lines at block edges are shared inside one L2.
+__device__ inline uint32_t xcd_block(uint32_t pid, uint32_t G)
+{
+    const uint32_t nx = 8;
+    uint32_t x = pid % nx, local = pid / nx;
+    uint32_t base = G / nx, r = G % nx;
+    return x * base + (x < r ? x : r) + local;
+}
+
+// pull source for a face cell (kernel.cl:328-575 upwindID_B): wrap across non-wall faces,
+// -1 for a link through a wall face.
+__device__ inline long face_upwind(int q, int i, int j, int k, int nx, int ny, int nz, int b1, int b2, int b3)
+{
+    int si = i - L19::cx[q], sj = j - L19::cy[q], sk = k - L19::cz[q];
+    if (si < 0 || si >= nx) {
+        if (is_wall_code(b1)) return -1;
+        si = (si < 0) ? nx - 1 : 0;
+    }
+    if (sj < 0 || sj >= ny) {
+        if (is_wall_code(b2)) return -1;
+        sj = (sj < 0) ? ny - 1 : 0;
+    }
+    if (sk < 0 || sk >= nz) {
+        if (is_wall_code(b3)) return -1;
+        sk = (sk < 0) ? nz - 1 : 0;
+    }
+    return (long)si + (long)nx * ((long)sj + (long)ny * sk);
+}
+
+// kernel.cl:105-326 reflectOrMirrorQ as a table lookup: bounce-back partner, with
+// Symmetry faces flipping only the normal component of diagonal directions.
+__device__ inline int mirror_q(int q, int b1, int b2, int b3)
+{
+    if (q <= 6) return L19::opp(q);
+    int cx = L19::cx[q], cy = L19::cy[q], cz = L19::cz[q];
+    // flip the component normal to the first Symmetry face the reference tests
+    if (q <= 10) {  // xy diagonals: test b1 then b2
+        if (b1 == BC_SYMMETRY) cx = -cx;
+        else if (b2 == BC_SYMMETRY) cy = -cy;
+        else { cx = -cx; cy = -cy; }
+    } else if (q <= 14) {  // xz diagonals: b1 then b3
+        if (b1 == BC_SYMMETRY) cx = -cx;
+        else if (b3 == BC_SYMMETRY) cz = -cz;
+        else { cx = -cx; cz = -cz; }
+    } else {  // yz diagonals: b2 then b3
+        if (b2 == BC_SYMMETRY) cy = -cy;
+        else if (b3 == BC_SYMMETRY) cz = -cz;
+        else { cy = -cy; cz = -cz; }
+    }
+#pragma unroll
+    for (int p = 7; p < 19; p++)
+        if (L19::cx[p] == cx && L19::cy[p] == cy && L19::cz[p] == cz) return p;
+    return L19::opp(q);
+}
+
+// kernel.cl:1346-1395 cornerFlag
+__device__ inline int corner_flag(int b3, int i, int j, int k, int nx, int ny, int nz)
+{
+    bool xi = (i == 0 || i == nx - 1), yj = (j == 0 || j == ny - 1), zk = (k == 0 || k == nz - 1);
+    if (nz - 1 == 0 || b3 == 0) return (xi && yj) ? 1 : 0;
+    return ((xi && yj) || (xi && zk) || (yj && zk)) ? 1 : 0;
+}
+
+// kernel.cl:1621-1652 calcTauw (Newton on the wall law; the loop's error test uses
+// the initial y+ as the reference does, K19:1646)
+template <typename Real>
+__device__ inline Real calc_tauw(Real rhow, Real magUp, Real nu)
+{
+    const Real y = Real(1.5), kappa = Real(0.42), E = Real(9.1);
+    Real ut = sqrt(nu * magUp);
+    int iter = 0;
+    Real err;
+    const Real yPlus = y * ut / nu;
+    do {
+        Real kUu = fmin(kappa * magUp / ut, Real(50));
+        Real fkUu = exp(kUu) - Real(1) - kUu * (Real(1) + Real(0.5) * kUu);
+        Real func = -ut * y / nu + magUp / ut + (fkUu - kUu * kUu * kUu / Real(6)) / E;
+        Real dfunc = y / nu + magUp / (ut * ut) + (kUu * fkUu / ut) / E;
+        ut = ut + func / dfunc;
+        err = fabs(yPlus - y * ut / nu);
+    } while (err > Real(0.005) && ++iter < 100);
+    ut = fmax(ut, Real(0));
+    return rhow * ut * ut;
+}
+
+// sum of the populations that leave through a face plus twice those arriving (Zou-He
+// wall density numerator); dir 0/1/2 = x/y/z, lo = min face
+template <typename Real>
+__device__ inline Real zouhe_sum(const Real *ft, int dir, bool lo)
+{
+    Real s = Real(0), t = Real(0);
+    // tangential populations in the reference's summation order
+    if (dir == 0) {
+        s = ft[0] + ft[3] + ft[4] + ft[5] + ft[6] + ft[15] + ft[16] + ft[17] + ft[18];
+        t = lo ? (ft[2] + ft[10] + ft[8] + ft[14] + ft[12]) : (ft[1] + ft[7] + ft[9] + ft[11] + ft[13]);
+    } else if (dir == 1) {
+        s = ft[0] + ft[1] + ft[2] + ft[5] + ft[6] + ft[11] + ft[12] + ft[14] + ft[13];
+        t = lo ? (ft[4] + ft[9] + ft[8] + ft[18] + ft[16]) : (ft[3] + ft[7] + ft[10] + ft[15] + ft[17]);
+    } else {
+        s = ft[0] + ft[1] + ft[2] + ft[3] + ft[4] + ft[7] + ft[8] + ft[9] + ft[10];
+        t = lo ? (ft[6] + ft[12] + ft[13] + ft[16] + ft[17]) : (ft[5] + ft[14] + ft[11] + ft[18] + ft[15]);
+    }
+    return s + Real(2) * t;
+}
+
+// Ladd moving-wall correction of the five populations entering through a face with
+// wall velocity (un, ut1, ut2) = (normal, first tangential, second tangential),
+// kernel.cl:1397-1512.  Written with value selects (never "if (lo) ft[a] else ft[b]")
+// so that ft[] stays in registers.
+template <int A0, int A1, int A2, int A3, int A4, int B0, int B1, int B2, int B3, int B4, typename Real>
+__device__ inline void ladd_face(Real *ft, bool lo, Real rhow, Real un, Real ut1, Real ut2)
+{
+    const Real d0 = rhow * un / Real(3);
+    const Real d1 = rhow * (un + ut1) / Real(6);
+    const Real d2 = rhow * (un - ut1) / Real(6);
+    const Real d3 = rhow * (un + ut2) / Real(6);
+    const Real d4 = rhow * (un - ut2) / Real(6);
+    ft[A0] = lo ? ft[A0] + d0 : ft[A0];
+    ft[A1] = lo ? ft[A1] + d1 : ft[A1];
+    ft[A2] = lo ? ft[A2] + d2 : ft[A2];
+    ft[A3] = lo ? ft[A3] + d3 : ft[A3];
+    ft[A4] = lo ? ft[A4] + d4 : ft[A4];
+    ft[B0] = lo ? ft[B0] : ft[B0] - d0;
+    ft[B1] = lo ? ft[B1] : ft[B1] - d1;
+    ft[B2] = lo ? ft[B2] : ft[B2] - d2;
+    ft[B3] = lo ? ft[B3] : ft[B3] - d3;
+    ft[B4] = lo ? ft[B4] : ft[B4] - d4;
+}
+
+template <typename Real>
+__device__ inline void ladd(Real *ft, int dir, bool lo, Real rhow, Real un, Real ut1, Real ut2)
+{
+    if (dir == 0)  // tangentials v, w
+        ladd_face<1, 7, 9, 11, 13, 2, 8, 10, 12, 14>(ft, lo, rhow, un, ut1, ut2);
+    else if (dir == 1)  // tangentials u, w
+        ladd_face<3, 7, 10, 15, 17, 4, 8, 9, 16, 18>(ft, lo, rhow, un, ut1, ut2);
+    else  // tangentials u, v
+        ladd_face<5, 11, 14, 15, 18, 6, 12, 13, 16, 17>(ft, lo, rhow, un, ut1, ut2);
+}
+
+template <typename Real>
+struct Cell {
+    long ic;
+    int i, j, k;
+};
+
+// macro value of a neighbour: previous step's buffer; the cell itself: its new value
+template <typename Real>
+__device__ inline Real mac_at(const Real *old, long idx, long ic, Real self)
+{
+    return idx == ic ? self : old[idx];
+}
+
+// one component of grad(u) as kernel.cl:2275-2304 forms it
+template <typename Real>
+__device__ inline Real grad_comp(const Real *old, Real self, long ic, int i, int j, int k, int nx, int ny, int nz,
+                                 int dir)
+{
+    const long sx = 1, sy = nx, sz = (long)nx * ny;
+    const long base = ic - i * sx - j * sy - k * sz;
+#define AT(ii, jj, kk) mac_at<Real>(old, base + (ii) * sx + (jj) * sy + (long)(kk) * sz, ic, self)
+    if (dir == 0)
+        return (i > 0 && i < nx - 1) ? Real(0.5) * (AT(i + 1, j, k) - AT(i - 1, j, k))
+             : (i == 0) ? AT(1, j, k) - AT(0, j, k)
+                        : AT(nx - 1, j, k) - AT(nx - 2, j, k);
+    if (dir == 1)
+        return (j > 0 && j < ny - 1) ? Real(0.5) * (AT(i, j + 1, k) - AT(i, j - 1, k))
+             : (j == 0) ? AT(i, 1, k) - AT(i, 0, k)
+                        : AT(i, ny - 1, k) - AT(i, ny - 2, k);
+    if (k > 0 && k < nz - 1) return Real(0.5) * (AT(i, j, k + 1) - AT(i, j, k - 1));
+    if (nz - 1 == 0) return Real(0);
+    if (k == 0) {
+        Real a = AT(i, j, 1);
+        return a - a;
+    }
+    return AT(i, j, nz - 1) - AT(i, j, nz - 2);
+#undef AT
+}
+
+// kernel.cl:2267-2365 waleTauSGS (Cw = 0.325)
+template <typename Real>
+__device__ inline Real wale_tau(const Step19Args<Real> &a, long ic, int i, int j, int k, Real u, Real v, Real w)
+{
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    Real dudx = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 0);
+    Real dudy = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 1);
+    Real dudz = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 2);
+    Real dvdx = grad_comp<Real>(a.v_o, v, ic, i, j, k, nx, ny, nz, 0);
+    Real dvdy = grad_comp<Real>(a.v_o, v, ic, i, j, k, nx, ny, nz, 1);
+    Real dvdz = grad_comp<Real>(a.v_o, v, ic, i, j, k, nx, ny, nz, 2);
+    Real dwdx = grad_comp<Real>(a.w_o, w, ic, i, j, k, nx, ny, nz, 0);
+    Real dwdy = grad_comp<Real>(a.w_o, w, ic, i, j, k, nx, ny, nz, 1);
+    Real dwdz = grad_comp<Real>(a.w_o, w, ic, i, j, k, nx, ny, nz, 2);
+    const Real Cw = Real(0.325);
+    Real S12 = Real(0.5) * (dudy + dvdx), S13 = Real(0.5) * (dudz + dwdx), S23 = Real(0.5) * (dvdz + dwdy);
+    Real SS = dudx * dudx + Real(2) * S12 * S12 + Real(2) * S13 * S13 + dvdy * dvdy + Real(2) * S23 * S23
+              + dwdz * dwdz;
+    // g = grad(u)^2 (g_ab = sum_c d_c u_a ... as the reference composes it)
+    Real g11 = dudx * dudx + dvdx * dudy + dwdx * dudz;
+    Real g12 = dudx * dvdx + dvdx * dvdy + dwdx * dvdz;
+    Real g13 = dudx * dwdx + dvdx * dwdy + dwdx * dwdz;
+    Real g21 = dudy * dudx + dvdy * dudy + dwdy * dudz;
+    Real g22 = dudy * dvdx + dvdy * dvdy + dwdy * dvdz;
+    Real g23 = dudy * dwdx + dvdy * dwdy + dwdy * dwdz;
+    Real g31 = dudz * dudx + dvdz * dudy + dwdz * dudz;
+    Real g32 = dudz * dvdx + dvdz * dvdy + dwdz * dvdz;
+    Real g33 = dudz * dwdx + dvdz * dwdy + dwdz * dwdz;
+    Real tr3 = (g11 + g22 + g33) / Real(3);
+    Real d11 = g11 - tr3, d22 = g22 - tr3, d33 = g33 - tr3;
+    Real d12 = Real(0.5) * (g12 + g21), d13 = Real(0.5) * (g13 + g31), d23 = Real(0.5) * (g23 + g32);
+    Real SdSd = d11 * d11 + d22 * d22 + d33 * d33 + Real(2) * (d12 * d12 + d13 * d13 + d23 * d23);
+    if (!(SS > Real(1e-24) && SdSd > Real(1e-24))) return Real(0);
+    Real sd15 = SdSd * sqrt(SdSd);          // SdSd^1.5
+    Real ss25 = SS * SS * sqrt(SS);         // SS^2.5
+    Real sd125 = SdSd * sqrt(sqrt(SdSd));        // SdSd^1.25
+    return Real(9) * (Cw * Cw * sd15 / (ss25 + sd125));
+}
+
+// kernel.cl:2133-2201 smagorinskyTauSGS; Pi_neq from the incompressible-form feq
+template <typename Real>
+__device__ inline Real smagorinsky_tau(const Real *ft, Real rho, Real u, Real v, Real w, Real omega, Real LES,
+                                       Real sdf, Real Cs, int damping)
+{
+    const Real tau = Real(1) / omega;
+    Real Pxx = 0, Pxy = 0, Pxz = 0, Pyy = 0, Pyz = 0, Pzz = 0;
+    const Real uSqr = u * u + v * v + w * w;
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const Real cx = Real(L19::cx[q]), cy = Real(L19::cy[q]), cz = Real(L19::cz[q]);
+        const Real uDotC = u * cx + v * cy + w * cz;
+        const Real feq = (rho + Real(3) * uDotC + Real(4.5) * uDotC * uDotC - Real(1.5) * uSqr) * w19<Real>(q);
+        const Real d = ft[q] - feq;
+        if (L19::cx[q] != 0) Pxx += d;
+        if (L19::cy[q] != 0) Pyy += d;
+        if (L19::cz[q] != 0) Pzz += d;
+        if (L19::cx[q] * L19::cy[q] != 0) Pxy += Real(L19::cx[q] * L19::cy[q]) * d;
+        if (L19::cx[q] * L19::cz[q] != 0) Pxz += Real(L19::cx[q] * L19::cz[q]) * d;
+        if (L19::cy[q] * L19::cz[q] != 0) Pyz += Real(L19::cy[q] * L19::cz[q]) * d;
+    }
+    const Real PP = sqrt(Pxx * Pxx + Pyy * Pyy + Pzz * Pzz + Real(2) * (Pxy * Pxy + Pxz * Pxz + Pyz * Pyz));
+    Real t = LES * Real(0.5) * (-tau + sqrt(tau * tau + Real(18) * Real(1.4142135623730951) * Cs * Cs * PP / rho));
+    if (damping == 1 && sdf != Real(10000) && sdf > Real(0)) {
+        const Real m = fmin(Real(1), Real(0.41) * sdf / Cs);
+        t *= m * m;
+    }
+    return t;
+}
+
+// kernel.cl:2444-2503 tauSpongeZone
+template <typename Real>
+__device__ inline Real sponge_tau(const Step19Args<Real> &a, Real omega, Real tauSGS, int i, int j, int k)
+{
+    Real tau = Real(1) / omega;
+    const Real spz = a.spz;
+    const int m = a.sponge_mask;
+    if (m == 0) return tau;
+    const Real one_m = Real(1) - tauSGS;
+    if ((m & 1) && Real(i) < Real(a.nx) * spz) {
+        const Real fx = Real(i) / (Real(a.nx) * spz);
+        tau = (Real(1) - fx) * one_m + fx * tau;
+    }
+    if ((m & 2) && Real(i) > Real(a.nx) * (Real(1) - spz)) {
+        const Real fx = Real(a.nx - i) / (Real(a.nx) * spz);
+        tau = (Real(1) - fx) * one_m + fx * tau;
+    }
+    if ((m & 4) && Real(j) < Real(a.ny) * spz) {
+        const Real fx = Real(j) / (Real(a.ny) * spz);
+        tau = (Real(1) - fx) * one_m + fx * tau;
+    }
+    if ((m & 8) && Real(j) > Real(a.ny) * (Real(1) - spz)) {
+        const Real fx = Real(a.ny - j) / (Real(a.ny) * spz);
+        tau = (Real(1) - fx) * one_m + fx * tau;
+    }
+    if ((m & 16) && Real(k) < Real(a.nz) * spz) {
+        const Real fx = Real(k) / (Real(a.nz) * spz);
+        tau = (Real(1) - fx) * one_m + fx * tau;
+    }
+    if ((m & 32) && Real(k) > Real(a.nz) * (Real(1) - spz)) {
+        const Real fx = Real(a.nz - k) / (Real(a.nz) * spz);
+        tau = (Real(1) - fx) * one_m + fx * tau;
+    }
+    return tau;
+}
+
+// kernel.cl:3156-3309 recursive-regularized collision (bulk/shear split of the
+// 2nd-order non-equilibrium moments, 3rd/4th order rebuilt recursively), with the
+// body force S_q = rho w_q 3 c_qx dpdx.  u is the force-shifted velocity.
+template <typename Real>
+__device__ inline void collide_rr(Real *out, const Real *ft, Real rho, Real u, Real v, Real w, Real tau,
+                                  Real tauSGS, Real omegaB, Real dpdx)
+{
+    const Real cs2 = Real(1) / Real(3), cs4 = Real(1) / Real(9);
+    const Real omegaEff = Real(1) / (tau + tauSGS);
+    // second-order Hermite moments of ft
+    const Real sxx = ft[1] + ft[2] + ft[7] + ft[8] + ft[9] + ft[10] + ft[11] + ft[12] + ft[13] + ft[14];
+    const Real syy = ft[3] + ft[4] + ft[7] + ft[8] + ft[9] + ft[10] + ft[15] + ft[16] + ft[17] + ft[18];
+    const Real szz = ft[5] + ft[6] + ft[11] + ft[12] + ft[13] + ft[14] + ft[15] + ft[16] + ft[17] + ft[18];
+    const Real invRho = Real(1) / rho;
+    const Real R200 = (sxx - cs2 * rho) * invRho;
+    const Real R020 = (syy - cs2 * rho) * invRho;
+    const Real R002 = (szz - cs2 * rho) * invRho;
+    const Real R110 = (ft[7] + ft[8] - ft[9] - ft[10]) * invRho;
+    const Real R101 = (ft[11] + ft[12] - ft[13] - ft[14]) * invRho;
+    const Real R011 = (ft[15] + ft[16] - ft[17] - ft[18]) * invRho;
+
+    const Real E200 = u * u, E020 = v * v, E002 = w * w, E110 = u * v, E101 = u * w, E011 = v * w;
+    const Real N200 = R200 - E200, N020 = R020 - E020, N002 = R002 - E002;
+    const Real N110 = R110 - E110, N101 = R101 - E101, N011 = R011 - E011;
+    const Real N210 = v * N200 + Real(2) * u * N110;
+    const Real N201 = w * N200 + Real(2) * u * N101;
+    const Real N021 = w * N020 + Real(2) * v * N011;
+    const Real N120 = u * N020 + Real(2) * v * N110;
+    const Real N102 = u * N002 + Real(2) * w * N101;
+    const Real N012 = v * N002 + Real(2) * w * N011;
+    const Real N220 = E020 * N200 + E200 * N020 + Real(4) * E110 * N110;
+    const Real N202 = E002 * N200 + E200 * N002 + Real(4) * E101 * N101;
+    const Real N022 = E002 * N020 + E020 * N002 + Real(4) * E011 * N011;
+
+    const Real omegaM = (omegaB - omegaEff) * cs2;
+    const Real omegaP = omegaM + omegaEff;
+    const Real C200 = R200 - omegaP * N200 - omegaM * N020 - omegaM * N002;
+    const Real C020 = R020 - omegaM * N200 - omegaP * N020 - omegaM * N002;
+    const Real C002 = R002 - omegaM * N200 - omegaM * N020 - omegaP * N002;
+    const Real C110 = R110 - omegaEff * N110;
+    const Real C101 = R101 - omegaEff * N101;
+    const Real C011 = R011 - omegaEff * N011;
+    const Real om1 = Real(1) - omegaEff;
+    const Real M210 = E200 * v + om1 * N210 + cs2 * v;
+    const Real M201 = E200 * w + om1 * N201 + cs2 * w;
+    const Real M021 = E020 * w + om1 * N021 + cs2 * w;
+    const Real M120 = E020 * u + om1 * N120 + cs2 * u;
+    const Real M102 = E002 * u + om1 * N102 + cs2 * u;
+    const Real M012 = E002 * v + om1 * N012 + cs2 * v;
+    const Real M220 = E200 * E020 + om1 * N220 + cs2 * (C200 + C020) + cs4;
+    const Real M202 = E200 * E002 + om1 * N202 + cs2 * (C200 + C002) + cs4;
+    const Real M022 = E020 * E002 + om1 * N022 + cs2 * (C020 + C002) + cs4;
+    const Real M200 = C200 + cs2, M020 = C020 + cs2, M002 = C002 + cs2;
+    const Real M110 = C110, M101 = C101, M011 = C011;
+
+    const Real h = Real(0.5) * rho, qr = Real(0.25) * rho;
+    const Real F1 = rho * w19<Real>(1) * Real(3) * dpdx;   // force on axis directions
+    const Real F7 = rho * w19<Real>(7) * Real(3) * dpdx;   // force on diagonals
+    const Real ax = h * (u + M200 - M120 - M102 - M220 - M202);
+    const Real ay = h * (v + M020 - M210 - M012 - M220 - M022);
+    const Real az = h * (w + M002 - M201 - M021 - M202 - M022);
+    out[0] = rho * (Real(1) - M200 - M020 - M002 + M220 + M202 + M022);
+    out[1] = ax + F1;
+    out[2] = rho * (-u + M120 + M102) + ax - F1;
+    out[3] = ay;
+    out[4] = rho * (-v + M210 + M012) + ay;
+    out[5] = az;
+    out[6] = rho * (-w + M201 + M021) + az;
+    const Real dxy = qr * (M110 + M210 + M120 + M220);
+    const Real dxz = qr * (M101 + M201 + M102 + M202);
+    const Real dyz = qr * (M011 + M021 + M012 + M022);
+    out[7] = dxy + F7;
+    out[8] = h * (-M210 - M120) + dxy - F7;
+    out[9] = h * (-M110 - M210) + dxy + F7;
+    out[10] = h * (-M110 - M120) + dxy - F7;
+    out[11] = dxz + F7;
+    out[12] = h * (-M201 - M102) + dxz - F7;
+    out[13] = h * (-M101 - M201) + dxz + F7;
+    out[14] = h * (-M101 - M102) + dxz - F7;
+    out[15] = dyz;
+    out[16] = h * (-M021 - M012) + dyz;
+    out[17] = h * (-M011 - M021) + dyz;
+    out[18] = h * (-M011 - M012) + dyz;
+}
+
+// Boundary handling of a domain-face cell: streaming with walls, then the face
+// conditions in the reference's order (kernel.cl:3581-3592).
+template <typename S, typename Real>
+__device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, uint32_t fl, long ic, int i, int j, int k,
+                                       Real omega)
+{
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const long N = a.N;
+    const S *fs = (const S *)a.fsrc;
+    const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
+    const bool sym = (b1 == BC_SYMMETRY || b2 == BC_SYMMETRY || b3 == BC_SYMMETRY);
+    Real Fx = 0, Fy = 0, Fz = 0;  // wall momentum of this step's domain-face links
+    ft[0] = Codec<S, Real, 19>::load(fs + ic, 0, w19<Real>(0));
+#pragma unroll
+    for (int q = 1; q < 19; q++) {
+        long src = face_upwind(q, i, j, k, nx, ny, nz, b1, b2, b3);
+        if (src >= 0) {
+            ft[q] = Codec<S, Real, 19>::load(fs + (long)q * N + src, q, w19<Real>(q));
+        } else {
+            int qb = sym ? mirror_q(q, b1, b2, b3) : L19::opp(q);
+            ft[q] = Codec<S, Real, 19>::load(fs + (long)qb * N + ic, qb, w19<Real>(qb));
+            Fx += Real(-2) * ft[q] * Real(L19::cx[q]);
+            Fy += Real(-2) * ft[q] * Real(L19::cy[q]);
+            Fz += Real(-2) * ft[q] * Real(L19::cz[q]);
+        }
+    }
+    const int corner = corner_flag(b3, i, j, k, nx, ny, nz);
+    const long sy = nx, sz = (long)nx * ny;
+
+    // fixedVelocityBC, kernel.cl:1397-1512
+    if (corner == 0 && (b1 == BC_FIXEDVELOCITY || b2 == BC_FIXEDVELOCITY || b3 == BC_FIXEDVELOCITY)) {
+        const Real u0 = a.u0[ic], v0 = a.v0[ic], w0 = a.w0[ic];
+        if (b1 == BC_FIXEDVELOCITY && (i == 0 || i == nx - 1)) {
+            const bool lo = (i == 0);
+            Real rhow = zouhe_sum(ft, 0, lo) / (lo ? Real(1) - u0 : Real(1) + u0);
+            if (lo && u0 < Real(0)) rhow = a.rho_o[ic + 1];
+            if (!lo && u0 > Real(0)) rhow = a.rho_o[ic];  // own stored density (K19:1424-1428)
+            ladd(ft, 0, lo, rhow, u0, v0, w0);
+        }
+        if (b2 == BC_FIXEDVELOCITY && (j == 0 || j == ny - 1)) {
+            const bool lo = (j == 0);
+            Real rhow = zouhe_sum(ft, 1, lo) / (lo ? Real(1) - v0 : Real(1) + v0);
+            if (lo && v0 < Real(0)) rhow = a.rho_o[ic + sy];
+            if (!lo && v0 > Real(0)) rhow = a.rho_o[ic - sy];
+            ladd(ft, 1, lo, rhow, v0, u0, w0);
+        }
+        if (b3 == BC_FIXEDVELOCITY && (k == 0 || k == nz - 1)) {
+            const bool lo = (k == 0);
+            Real rhow = zouhe_sum(ft, 2, lo) / (lo ? Real(1) - w0 : Real(1) + w0);
+            if (lo && w0 < Real(0)) rhow = a.rho_o[ic + sz];
+            if (!lo && w0 > Real(0)) rhow = a.rho_o[ic - sz];
+            ladd(ft, 2, lo, rhow, w0, u0, v0);
+        }
+    }
+
+    // fixedDensityBC with rho_w = 1, kernel.cl:1514-1611
+    if (corner == 0 && (b1 == BC_FIXEDDENSITY || b2 == BC_FIXEDDENSITY || b3 == BC_FIXEDDENSITY)) {
+        const Real uc = a.u_o[ic], vc = a.v_o[ic], wc = a.w_o[ic];
+#pragma unroll
+        for (int dir = 0; dir < 3; dir++) {
+            const int code = dir == 0 ? b1 : (dir == 1 ? b2 : b3);
+            if (code != BC_FIXEDDENSITY) continue;
+            const int pos = dir == 0 ? i : (dir == 1 ? j : k);
+            const int n = dir == 0 ? nx : (dir == 1 ? ny : nz);
+            const long st = dir == 0 ? 1 : (dir == 1 ? sy : sz);
+            // inner neighbour: index 1 at the min face, n-2 otherwise (kernel.cl:1520, 1560, 1590)
+            const long inner = (pos == 0) ? ic + st : ic + (long)(n - 2 - pos) * st;
+            const Real ue = uc + Real(0.5) * (uc - a.u_o[inner]);
+            const Real ve = vc + Real(0.5) * (vc - a.v_o[inner]);
+            const Real we = wc + Real(0.5) * (wc - a.w_o[inner]);
+            const Real uSqr = ue * ue + ve * ve + we * we;
+            const bool atlo = (pos == 0), athi = (pos == n - 1);
+#pragma unroll
+            for (int q = 1; q < 19; q++) {
+                const int cn = dir == 0 ? L19::cx[q] : (dir == 1 ? L19::cy[q] : L19::cz[q]);
+                if ((atlo && cn == 1) || (athi && cn == -1)) {
+                    const Real uDotC = ue * Real(L19::cx[q]) + ve * Real(L19::cy[q]) + we * Real(L19::cz[q]);
+                    ft[q] = -ft[q] + Real(2) * w19<Real>(q) * Real(1)
+                                         * (Real(1) + Real(4.5) * uDotC * uDotC - Real(1.5) * uSqr);
+                }
+            }
+        }
+    }
+
+    // wallFunctionBC, kernel.cl:1654-1924 (stationary wall, tangential slip velocity)
+    if (corner == 0 && (b1 == BC_WALLFUNCTION || b2 == BC_WALLFUNCTION || b3 == BC_WALLFUNCTION)) {
+        const Real nu = (Real(1) / omega - Real(0.5)) / Real(3);
+#pragma unroll
+        for (int dir = 0; dir < 3; dir++) {
+            const int code = dir == 0 ? b1 : (dir == 1 ? b2 : b3);
+            const int pos = dir == 0 ? i : (dir == 1 ? j : k);
+            const int n = dir == 0 ? nx : (dir == 1 ? ny : nz);
+            if (code != BC_WALLFUNCTION || !(pos == 0 || pos == n - 1)) continue;
+            const bool lo = (pos == 0);
+            const long st = dir == 0 ? 1 : (dir == 1 ? sy : sz);
+            const Real rhow = zouhe_sum(ft, dir, lo);
+            // neighbour sampled for the wall law: the next cell inward; the z faces of a
+            // 2-D lattice sample the cell itself (K19:1848)
+            long nb = lo ? ic + st : ic - st;
+            if (dir == 2) nb = (nz - 1 != 0) ? ic + st : ic;
+            const Real *t1 = dir == 0 ? a.v_o : a.u_o;
+            const Real *t2 = dir == 2 ? a.v_o : a.w_o;
+            Real p = t1[nb], s = t2[nb];
+            Real magUp = sqrt(p * p + s * s);
+            Real ut1 = 0, ut2 = 0;
+            if (magUp != Real(0)) {
+                const Real tauw = calc_tauw<Real>(rhow, magUp, nu);
+                p = t1[ic];
+                s = t2[ic];
+                magUp = sqrt(p * p + s * s);
+                if ((dir == 2 && lo) || magUp != Real(0)) {
+                    const Real e1 = p / magUp, e2 = s / magUp;
+                    const Real Fw1 = dir == 0 ? Fy : Fx;
+                    const Real Fw2 = dir == 2 ? Fy : Fz;
+                    ut1 = Real(-3) * (tauw * e1 - Fw1) / rhow;
+                    ut2 = Real(-3) * (tauw * e2 - Fw2) / rhow;
+                }
+            }
+            ladd(ft, dir, lo, rhow, Real(0), ut1, ut2);
+        }
+    }
+
+    // outflowBC, kernel.cl:2005-2097: convective outlet with the cell's last velocity
+    if (b1 == BC_OUTLET || b2 == BC_OUTLET || b3 == BC_OUTLET) {
+        const Real is3 = Real(0.57735026918962576);
+        const Real uo = a.u_o[ic], vo = a.v_o[ic], wo = a.w_o[ic];
+#pragma unroll
+        for (int q = 1; q < 19; q++) {
+            const int cx = L19::cx[q], cy = L19::cy[q], cz = L19::cz[q];
+            const S *fq = fs + (long)q * N;
+#define OUTF(cond, inner, un)                                                                          \
+    if (cond) {                                                                                        \
+        const Real fi = Codec<S, Real, 19>::load(fq + (inner), q, w19<Real>(q));                      \
+        const Real fc = Codec<S, Real, 19>::load(fq + ic, q, w19<Real>(q));                           \
+        ft[q] = (is3 - (un)) * fi + (Real(1) - (is3 - (un))) * fc;                                      \
+    }
+            OUTF(i == 0 && b1 == BC_OUTLET && cx == 1, ic + 1, uo)
+            OUTF(i == nx - 1 && b1 == BC_OUTLET && cx == -1, ic - 1, uo)
+            OUTF(j == 0 && b2 == BC_OUTLET && cy == 1, ic + sy, vo)
+            OUTF(j == ny - 1 && b2 == BC_OUTLET && cy == -1, ic - sy, vo)
+            OUTF(k == 0 && b3 == BC_OUTLET && cz == 1, ic + sz, wo)
+            OUTF(k == nz - 1 && b3 == BC_OUTLET && cz == -1, ic - sz, wo)
+#undef OUTF
+        }
+    }
+}
+
+template <typename S, typename Real, int SGS>
+__global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
+{
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t icu = blk * blockDim.x + threadIdx.x;
+    if (icu >= a.N) return;
+    const uint32_t fl = a.flag[icu];
+    if (fl & F_SOLID) return;
+    const long ic = icu;
+    const long N = a.N;
+    const uint32_t t = fdiv(icu, a.dnx);
+    const int i = (int)(icu - t * (uint32_t)a.nx);
+    const uint32_t kk = fdiv(t, a.dny);
+    const int j = (int)(t - kk * (uint32_t)a.ny);
+    const int k = (int)kk;
+    const S *fs = (const S *)a.fsrc;
+    S *fd = (S *)a.fdst;
+
+    const Real omega = (fl & F_OMEGAVAR) ? a.omega[ic] : a.omega0;
+    Real ft[19];
+    if (!(fl & F_FACE)) {
+#pragma unroll
+        for (int q = 0; q < 19; q++) ft[q] = Codec<S, Real, 19>::load(fs + (long)q * N + (ic - a.off[q]), q, w19<Real>(q));
+    } else {
+        face_cell<S, Real>(a, ft, fl, ic, i, j, k, omega);
+    }
+
+    // internalWallBC, kernel.cl:1926-2003: simple bounce-back on solid links
+    Real Fx = 0, Fy = 0, Fz = 0;
+    if (fl & F_NEISOLID) {
+        const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
+#pragma unroll
+        for (int q = 1; q < 19; q++) {
+            const long src = (fl & F_FACE) ? face_upwind(q, i, j, k, a.nx, a.ny, a.nz, b1, b2, b3) : ic - a.off[q];
+            if (src >= 0 && (a.flag[src] & F_SOLID)) {
+                const int qb = L19::opp(q);
+                const Real fb = Codec<S, Real, 19>::load(fs + (long)qb * N + ic, qb, w19<Real>(qb));
+                ft[q] = fb;
+                Fx += -(fb + fb) * Real(L19::cx[q]);
+                Fy += -(fb + fb) * Real(L19::cy[q]);
+                Fz += -(fb + fb) * Real(L19::cz[q]);
+            }
+        }
+    }
+
+    // updateRhoUVW, kernel.cl:3509-3523
+    Real rho = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        rho += ft[q];
+        if (L19::cx[q]) mx += Real(L19::cx[q]) * ft[q];
+        if (L19::cy[q]) my += Real(L19::cy[q]) * ft[q];
+        if (L19::cz[q]) mz += Real(L19::cz[q]) * ft[q];
+    }
+    const Real u = mx / rho + Real(0.5) * a.dpdx / rho;
+    const Real v = my / rho, w = mz / rho;
+    const bool mac = a.mac_all || (fl & F_KEEPMAC);
+    if (mac) {
+        a.rho_n[ic] = rho;
+        a.u_n[ic] = u;
+        a.v_n[ic] = v;
+        a.w_n[ic] = w;
+    }
+    if (a.aux_all || (fl & F_NEISOLID)) {
+        a.Fwx[ic] = Fx;
+        a.Fwy[ic] = Fy;
+        a.Fwz[ic] = Fz;
+    }
+    if (a.aux_all) {
+        a.Gx[ic] = Real(0);
+        a.Gy[ic] = Real(0);
+        a.Gz[ic] = Real(0);
+    }
+
+    Real tauSGS;
+    if constexpr (SGS == 0) {
+        tauSGS = wale_tau<Real>(a, ic, i, j, k, u, v, w);
+    } else if constexpr (SGS == 1) {
+        const Real sdf = (fl & F_HASSDF) ? a.sdf[ic] : Real(10000);
+        tauSGS = smagorinsky_tau<Real>(ft, rho, u, v, w, omega, a.LES, sdf, a.Cs, a.damping);
+    } else {
+        tauSGS = Real(0);
+    }
+    if (a.aux_all) a.tau[ic] = tauSGS;
+
+    const Real tau = sponge_tau<Real>(a, omega, tauSGS, i, j, k);
+    Real out[19];
+    collide_rr<Real>(out, ft, rho, u - Real(0.5) * a.dpdx / rho, v, w, tau, tauSGS, a.omegaB, a.dpdx);
+#pragma unroll
+    for (int q = 0; q < 19; q++) Codec<S, Real, 19>::store(fd + (long)q * N + ic, out[q], q, w19<Real>(q));
+}
+
+}  // namespace lbffs

@@ -1,0 +1,671 @@
+// lbffs_api.hip -- the C ABI of the engine (include/lbffs.h): device context, field
+// transfer, flag packing with input validation, the step launcher and FP64
+// reductions.  Replaces the OpenCL host section of D3Q19_opencl/ns.cpp:382-603.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbffs.h"
+#include "d3q19_step.hpp"
+
+using namespace lbffs;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                    \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return fail(LBFFS_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+// validation failures reported by the pack kernel (bit set in a device word)
+enum : int {
+    PACK_BAD_CODE = 1,         // face code outside 0..7
+    PACK_FACE_NOT_FLAGGED = 2, // lattice-face fluid cell with boundary==0 (its pulls would leave the lattice)
+    PACK_WALLFN_ZMAX = 4,      // WallFunction on the nzMax face: the reference samples k+1 there
+    PACK_2D_ZCODE = 8,         // nz==1 with a z-face condition that samples the layer k=1
+    PACK_BAD_BOUNDARY = 16,    // boundary value other than 0/1
+};
+
+}  // namespace
+
+struct lbffs_ctx {
+    lbffs_desc d;
+    long N = 0;
+    int real_size = 4;   // bytes of the host/compute real type
+    int store_size = 4;  // bytes of one stored population
+    void *f[2] = {nullptr, nullptr};
+    int cur = 0;
+    int32_t *boundary = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
+    uint8_t *solid = nullptr, *neiSolid = nullptr;
+    void *sdf = nullptr, *u0 = nullptr, *v0 = nullptr, *w0 = nullptr, *omega = nullptr;
+    void *Fwx = nullptr, *Fwy = nullptr, *Fwz = nullptr, *tau = nullptr;
+    void *Gx = nullptr, *Gy = nullptr, *Gz = nullptr;
+    void *rho[2] = {nullptr, nullptr}, *u[2] = {nullptr, nullptr}, *v[2] = {nullptr, nullptr},
+         *w[2] = {nullptr, nullptr};
+    int mcur = 0;
+    uint16_t *flag = nullptr;
+    int *d_err = nullptr;
+    double *d_part = nullptr;  // reduction partials
+    double *d_red = nullptr;
+    bool flags_dirty = true;
+    bool f_loaded = false;
+    int sponge_mask = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+};
+
+namespace {
+
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+// ---------------------------------------------------------------- kernels
+
+template <typename Real>
+__global__ void pack_flags_kernel(uint32_t N, int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
+                                  const int32_t *boundary, const int32_t *b1, const int32_t *b2, const int32_t *b3,
+                                  const uint8_t *solid, const uint8_t *neiSolid, const Real *sdf, const Real *omega,
+                                  Real omega0, const Real *w0, uint16_t *flag, int *err)
+{
+    uint32_t ic = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic >= N) return;
+    uint32_t t = fdiv(ic, dnx);
+    int i = (int)(ic - t * (uint32_t)nx);
+    uint32_t kk = fdiv(t, dny);
+    int j = (int)(t - kk * (uint32_t)ny);
+    int k = (int)kk;
+    int e = 0;
+    uint32_t fl = 0;
+    const int c1 = b1[ic], c2 = b2[ic], c3 = b3[ic];
+    const int bnd = boundary[ic];
+    const bool sol = solid[ic] != 0;
+    if (c1 < 0 || c1 > 7 || c2 < 0 || c2 > 7 || c3 < 0 || c3 > 7) e |= PACK_BAD_CODE;
+    if (bnd != 0 && bnd != 1) e |= PACK_BAD_BOUNDARY;
+    const bool geoface = (i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || (nz > 1 && (k == 0 || k == nz - 1)));
+    if (sol) fl |= F_SOLID;
+    if (bnd == 1) fl |= F_FACE;
+    if (!sol) {
+        if (geoface && bnd != 1) e |= PACK_FACE_NOT_FLAGGED;
+        if (bnd == 1) {
+            if (nz > 1 && k == nz - 1 && c3 == BC_WALLFUNCTION) e |= PACK_WALLFN_ZMAX;
+            if (nz == 1 && (c3 == BC_OUTLET || c3 == BC_FIXEDDENSITY || (c3 == BC_FIXEDVELOCITY && w0[ic] < Real(0))))
+                e |= PACK_2D_ZCODE;
+        }
+    }
+    if (neiSolid[ic] == 1) fl |= F_NEISOLID;
+    const bool keep = (i <= 1 || i >= nx - 2 || j <= 1 || j >= ny - 2 || (nz > 1 && (k <= 1 || k >= nz - 2)));
+    if (keep) fl |= F_KEEPMAC;
+    if (sdf[ic] != Real(10000)) fl |= F_HASSDF;
+    if (omega[ic] != omega0) fl |= F_OMEGAVAR;
+    fl |= ((uint32_t)(c1 & 7) << F_B1_SHIFT) | ((uint32_t)(c2 & 7) << F_B2_SHIFT) | ((uint32_t)(c3 & 7) << F_B3_SHIFT);
+    flag[ic] = (uint16_t)fl;
+    if (e) atomicOr(err, e);
+}
+
+template <typename S, typename Real, int Q>
+__global__ void encode_kernel(uint32_t N, const Real *src, S *dst)
+{
+    uint32_t ic = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic >= N) return;
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        Codec<S, Real, Q>::store(dst + (size_t)q * N + ic, src[(size_t)q * N + ic], q, w19<Real>(q));
+}
+
+template <typename S, typename Real, int Q>
+__global__ void decode_kernel(uint32_t N, const S *src, Real *dst)
+{
+    uint32_t ic = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic >= N) return;
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        dst[(size_t)q * N + ic] = Codec<S, Real, Q>::load(src + (size_t)q * N + ic, q, w19<Real>(q));
+}
+
+__device__ inline double wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    return x;
+}
+
+// pass 1: fixed grid, grid-stride over cells, per-block partial sums (deterministic order)
+template <typename S, typename Real>
+__global__ void __launch_bounds__(kRedThreads) reduce_kernel(uint32_t N, const uint16_t *flag, const S *f,
+                                                             const Real *Fwx, const Real *Fwy, const Real *Fwz,
+                                                             const Real *Gx, const Real *Gy, const Real *Gz,
+                                                             double *part)
+{
+    double acc[LBFFS_REDUCE_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t ic = blockIdx.x * blockDim.x + threadIdx.x; ic < N; ic += gridDim.x * blockDim.x) {
+        if (flag[ic] & F_SOLID) continue;
+        acc[0] += (double)Fwx[ic];
+        acc[1] += (double)Fwy[ic];
+        acc[2] += (double)Fwz[ic];
+        acc[3] += (double)Gx[ic];
+        acc[4] += (double)Gy[ic];
+        acc[5] += (double)Gz[ic];
+        Real rho = 0;
+#pragma unroll
+        for (int q = 0; q < 19; q++) rho += Codec<S, Real, 19>::load(f + (size_t)q * N + ic, q, w19<Real>(q));
+        acc[6] += (double)rho - 1.0;
+        acc[7] += 1.0;
+    }
+    __shared__ double sh[kRedThreads / 64][LBFFS_REDUCE_N];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < LBFFS_REDUCE_N; c++) {
+        double s = wave_sum(acc[c]);
+        if (lane == 0) sh[wid][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < LBFFS_REDUCE_N) {
+        double s = 0;
+        for (int wv = 0; wv < kRedThreads / 64; wv++) s += sh[wv][threadIdx.x];
+        part[blockIdx.x * LBFFS_REDUCE_N + threadIdx.x] = s;
+    }
+}
+
+// pass 2: one block sums the partials in block order
+__global__ void reduce_final_kernel(const double *part, int nblocks, double *out)
+{
+    if (threadIdx.x < LBFFS_REDUCE_N) {
+        double s = 0;
+        for (int b = 0; b < nblocks; b++) s += part[b * LBFFS_REDUCE_N + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------- helpers
+
+bool is_f64(const lbffs_ctx *c) { return c->d.storage == LBFFS_STORE_F64; }
+
+size_t field_host_bytes(const lbffs_ctx *c, int field)
+{
+    const size_t N = (size_t)c->N;
+    switch (field) {
+    case LBFFS_FIELD_F: return (size_t)c->d.q * N * c->real_size;
+    case LBFFS_FIELD_BOUNDARY:
+    case LBFFS_FIELD_BOUNDARY1:
+    case LBFFS_FIELD_BOUNDARY2:
+    case LBFFS_FIELD_BOUNDARY3: return N * sizeof(int32_t);
+    case LBFFS_FIELD_SOLID:
+    case LBFFS_FIELD_NEISOLID: return N;
+    default: return N * c->real_size;
+    }
+}
+
+void *field_ptr(lbffs_ctx *c, int field)
+{
+    switch (field) {
+    case LBFFS_FIELD_BOUNDARY: return c->boundary;
+    case LBFFS_FIELD_BOUNDARY1: return c->b1;
+    case LBFFS_FIELD_BOUNDARY2: return c->b2;
+    case LBFFS_FIELD_BOUNDARY3: return c->b3;
+    case LBFFS_FIELD_SDF: return c->sdf;
+    case LBFFS_FIELD_SOLID: return c->solid;
+    case LBFFS_FIELD_NEISOLID: return c->neiSolid;
+    case LBFFS_FIELD_U0: return c->u0;
+    case LBFFS_FIELD_V0: return c->v0;
+    case LBFFS_FIELD_W0: return c->w0;
+    case LBFFS_FIELD_FWX: return c->Fwx;
+    case LBFFS_FIELD_FWY: return c->Fwy;
+    case LBFFS_FIELD_FWZ: return c->Fwz;
+    case LBFFS_FIELD_TAUSGS: return c->tau;
+    case LBFFS_FIELD_RHO: return c->rho[c->mcur];
+    case LBFFS_FIELD_U: return c->u[c->mcur];
+    case LBFFS_FIELD_V: return c->v[c->mcur];
+    case LBFFS_FIELD_W: return c->w[c->mcur];
+    case LBFFS_FIELD_OMEGA: return c->omega;
+    case LBFFS_FIELD_GX: return c->Gx;
+    case LBFFS_FIELD_GY: return c->Gy;
+    case LBFFS_FIELD_GZ: return c->Gz;
+    }
+    return nullptr;
+}
+
+bool field_affects_flags(int field)
+{
+    switch (field) {
+    case LBFFS_FIELD_BOUNDARY:
+    case LBFFS_FIELD_BOUNDARY1:
+    case LBFFS_FIELD_BOUNDARY2:
+    case LBFFS_FIELD_BOUNDARY3:
+    case LBFFS_FIELD_SDF:
+    case LBFFS_FIELD_SOLID:
+    case LBFFS_FIELD_NEISOLID:
+    case LBFFS_FIELD_OMEGA:
+    case LBFFS_FIELD_W0: return true;
+    }
+    return false;
+}
+
+inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+int dalloc(void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(LBFFS_E_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    e = hipMemset(*p, 0, bytes);
+    if (e != hipSuccess) return fail(LBFFS_E_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
+    return 0;
+}
+
+template <typename Real>
+int pack_flags(lbffs_ctx *c)
+{
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    HIP_TRY(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+    Real om0 = (Real)c->d.omega0;
+    hipLaunchKernelGGL((pack_flags_kernel<Real>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
+                       (uint32_t)c->N, nx, ny, nz, make_fastdiv(nx), make_fastdiv(ny), c->boundary, c->b1, c->b2,
+                       c->b3, c->solid, c->neiSolid, (const Real *)c->sdf, (const Real *)c->omega, om0,
+                       (const Real *)c->w0, c->flag, c->d_err);
+    HIP_TRY(hipGetLastError());
+    int err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    // face-centre codes that switch the sponge on (kernel.cl:2446-2451)
+    int codes[6];
+    const long cX0 = (long)nx * ny * (nz / 2) + (long)nx * (ny / 2);
+    const long cXE = cX0 + nx - 1;
+    const long cY0 = (long)nx * ny * (nz / 2) + nx / 2;
+    const long cYE = cY0 + (long)nx * (ny - 1);
+    const long cZ0 = (long)nx * (ny / 2) + nx / 2;
+    const long cZE = cZ0 + (long)nx * ny * (nz - 1);
+    HIP_TRY(hipMemcpyAsync(&codes[0], c->b1 + cX0, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&codes[1], c->b1 + cXE, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&codes[2], c->b2 + cY0, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&codes[3], c->b2 + cYE, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&codes[4], c->b3 + cZ0, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&codes[5], c->b3 + cZE, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (err) {
+        std::string m = "invalid lattice description:";
+        if (err & PACK_BAD_CODE) m += " face code outside 0..7;";
+        if (err & PACK_BAD_BOUNDARY) m += " boundary value other than 0/1;";
+        if (err & PACK_FACE_NOT_FLAGGED) m += " fluid cell on a lattice face with boundary==0;";
+        if (err & PACK_WALLFN_ZMAX) m += " WallFunction on the nzMax face (the reference samples k+1 there);";
+        if (err & PACK_2D_ZCODE) m += " nz==1 with an Outlet/FixedDensity/inflowing FixedVelocity z face;";
+        return fail(LBFFS_E_INVALID, m);
+    }
+    c->sponge_mask = 0;
+    for (int s = 0; s < 6; s++)
+        if (codes[s] == BC_OUTLET || codes[s] == BC_FIXEDDENSITY) c->sponge_mask |= 1 << s;
+    c->flags_dirty = false;
+    return 0;
+}
+
+template <typename S, typename Real>
+void set_args(lbffs_ctx *c, Step19Args<Real> &a, const lbffs_step_params *p)
+{
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    a.N = (uint32_t)c->N;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.dnx = make_fastdiv(nx);
+    a.dny = make_fastdiv(ny);
+    for (int q = 0; q < 19; q++) {
+        const int zoff = (nz == 1) ? 0 : L19::cz[q];  // nz==1: z links stay in the layer (kernel.cl:907-989)
+        a.off[q] = L19::cx[q] + nx * L19::cy[q] + nx * ny * zoff;
+    }
+    a.flag = c->flag;
+    a.u0 = (const Real *)c->u0;
+    a.v0 = (const Real *)c->v0;
+    a.w0 = (const Real *)c->w0;
+    a.sdf = (const Real *)c->sdf;
+    a.omega = (const Real *)c->omega;
+    a.tau = (Real *)c->tau;
+    a.Fwx = (Real *)c->Fwx;
+    a.Fwy = (Real *)c->Fwy;
+    a.Fwz = (Real *)c->Fwz;
+    a.Gx = (Real *)c->Gx;
+    a.Gy = (Real *)c->Gy;
+    a.Gz = (Real *)c->Gz;
+    a.dpdx = (Real)p->dpdx;
+    a.LES = (Real)p->LES;
+    a.omegaB = (Real)p->omegaB;
+    a.spz = (Real)p->spzWidth;
+    a.Cs = (Real)p->Cs;
+    a.omega0 = (Real)c->d.omega0;
+    a.damping = p->damping;
+    a.sponge_mask = c->sponge_mask;
+}
+
+template <typename S, typename Real>
+int run_steps(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
+{
+    Step19Args<Real> a;
+    set_args<S, Real>(c, a, p);
+    const int block = c->d.block > 0 ? c->d.block : 256;
+    const unsigned grid = grid_for(c->N, block);
+    for (int s = 0; s < nsteps; s++) {
+        const bool last = (s == nsteps - 1);
+        a.fsrc = c->f[c->cur];
+        a.fdst = c->f[1 - c->cur];
+        a.rho_o = (const Real *)c->rho[c->mcur];
+        a.u_o = (const Real *)c->u[c->mcur];
+        a.v_o = (const Real *)c->v[c->mcur];
+        a.w_o = (const Real *)c->w[c->mcur];
+        a.rho_n = (Real *)c->rho[1 - c->mcur];
+        a.u_n = (Real *)c->u[1 - c->mcur];
+        a.v_n = (Real *)c->v[1 - c->mcur];
+        a.w_n = (Real *)c->w[1 - c->mcur];
+        a.aux_all = (p->store_every || last) ? 1 : 0;
+        a.mac_all = (a.aux_all || c->d.sgs == LBFFS_SGS_WALE) ? 1 : 0;
+        switch (c->d.sgs) {
+        case LBFFS_SGS_WALE:
+            hipLaunchKernelGGL((step19_kernel<S, Real, 0>), dim3(grid), dim3(block), 0, c->stream, a);
+            break;
+        case LBFFS_SGS_SMAGORINSKY:
+            hipLaunchKernelGGL((step19_kernel<S, Real, 1>), dim3(grid), dim3(block), 0, c->stream, a);
+            break;
+        default:
+            hipLaunchKernelGGL((step19_kernel<S, Real, 2>), dim3(grid), dim3(block), 0, c->stream, a);
+            break;
+        }
+        HIP_TRY(hipGetLastError());
+        c->cur ^= 1;
+        c->mcur ^= 1;
+    }
+    return 0;
+}
+
+template <typename S, typename Real>
+int upload_f(lbffs_ctx *c, const void *host)
+{
+    const size_t n = (size_t)c->d.q * c->N;
+    if (sizeof(S) == sizeof(Real)) {
+        HIP_TRY(hipMemcpy(c->f[c->cur], host, n * sizeof(Real), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->f[1 - c->cur], host, n * sizeof(Real), hipMemcpyHostToDevice));
+        return 0;
+    }
+    void *tmp = nullptr;
+    if (int e = dalloc(&tmp, n * sizeof(Real))) return e;
+    hipError_t e1 = hipMemcpy(tmp, host, n * sizeof(Real), hipMemcpyHostToDevice);
+    if (e1 == hipSuccess) {
+        for (int b = 0; b < 2; b++)
+            hipLaunchKernelGGL((encode_kernel<S, Real, 19>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
+                               (uint32_t)c->N, (const Real *)tmp, (S *)c->f[b]);
+        e1 = hipGetLastError();
+        if (e1 == hipSuccess) e1 = hipStreamSynchronize(c->stream);
+    }
+    hipFree(tmp);
+    if (e1 != hipSuccess) return fail(LBFFS_E_DEVICE, std::string("upload f: ") + hipGetErrorString(e1));
+    return 0;
+}
+
+template <typename S, typename Real>
+int download_f(lbffs_ctx *c, void *host)
+{
+    const size_t n = (size_t)c->d.q * c->N;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (sizeof(S) == sizeof(Real)) {
+        HIP_TRY(hipMemcpy(host, c->f[c->cur], n * sizeof(Real), hipMemcpyDeviceToHost));
+        return 0;
+    }
+    void *tmp = nullptr;
+    if (int e = dalloc(&tmp, n * sizeof(Real))) return e;
+    hipLaunchKernelGGL((decode_kernel<S, Real, 19>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
+                       (uint32_t)c->N, (const S *)c->f[c->cur], (Real *)tmp);
+    hipError_t e1 = hipGetLastError();
+    if (e1 == hipSuccess) e1 = hipStreamSynchronize(c->stream);
+    if (e1 == hipSuccess) e1 = hipMemcpy(host, tmp, n * sizeof(Real), hipMemcpyDeviceToHost);
+    hipFree(tmp);
+    if (e1 != hipSuccess) return fail(LBFFS_E_DEVICE, std::string("download f: ") + hipGetErrorString(e1));
+    return 0;
+}
+
+template <typename S, typename Real>
+int reduce_impl(lbffs_ctx *c, double *out)
+{
+    hipLaunchKernelGGL((reduce_kernel<S, Real>), dim3(kRedBlocks), dim3(kRedThreads), 0, c->stream, (uint32_t)c->N,
+                       c->flag, (const S *)c->f[c->cur], (const Real *)c->Fwx, (const Real *)c->Fwy,
+                       (const Real *)c->Fwz, (const Real *)c->Gx, (const Real *)c->Gy, (const Real *)c->Gz,
+                       c->d_part);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(64), 0, c->stream, c->d_part, kRedBlocks, c->d_red);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, c->d_red, sizeof(double) * LBFFS_REDUCE_N, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// dispatch on the storage form
+template <template <typename, typename> class F, typename... A>
+int dispatch(lbffs_ctx *c, A... args)
+{
+    switch (c->d.storage) {
+    case LBFFS_STORE_F32: return F<float, float>::run(c, args...);
+    case LBFFS_STORE_F16DEV: return F<__half, float>::run(c, args...);
+    case LBFFS_STORE_BF16DEV: return F<bf16_t, float>::run(c, args...);
+    case LBFFS_STORE_F64: return F<double, double>::run(c, args...);
+    }
+    return fail(LBFFS_E_UNSUPPORTED, "storage form");
+}
+
+template <typename S, typename Real>
+struct StepOp {
+    static int run(lbffs_ctx *c, int n, const lbffs_step_params *p) { return run_steps<S, Real>(c, n, p); }
+};
+template <typename S, typename Real>
+struct UploadFOp {
+    static int run(lbffs_ctx *c, const void *h) { return upload_f<S, Real>(c, h); }
+};
+template <typename S, typename Real>
+struct DownloadFOp {
+    static int run(lbffs_ctx *c, void *h) { return download_f<S, Real>(c, h); }
+};
+template <typename S, typename Real>
+struct ReduceOp {
+    static int run(lbffs_ctx *c, double *o) { return reduce_impl<S, Real>(c, o); }
+};
+
+void free_all(lbffs_ctx *c)
+{
+    void *ptrs[] = {c->f[0], c->f[1], c->boundary, c->b1, c->b2, c->b3, c->solid, c->neiSolid, c->sdf, c->u0, c->v0,
+                    c->w0, c->omega, c->Fwx, c->Fwy, c->Fwz, c->tau, c->Gx, c->Gy, c->Gz, c->rho[0], c->rho[1],
+                    c->u[0], c->u[1], c->v[0], c->v[1], c->w[0], c->w[1], c->flag, c->d_err, c->d_part, c->d_red};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out)
+{
+    if (!desc || !out) return fail(LBFFS_E_INVALID, "null argument");
+    *out = nullptr;
+    if (desc->abi_version != LBFFS_ABI_VERSION) return fail(LBFFS_E_INVALID, "abi_version mismatch");
+    if (desc->q != 19) return fail(LBFFS_E_UNSUPPORTED, "only q=19 is implemented in this build");
+    if (desc->nx < 2 || desc->ny < 2 || desc->nz < 1) return fail(LBFFS_E_INVALID, "lattice needs nx,ny >= 2, nz >= 1");
+    const long N = (long)desc->nx * desc->ny * desc->nz;
+    if (N >= (1L << 31) / 1 || (long)desc->q * N < 0) return fail(LBFFS_E_INVALID, "cell count must be below 2^31");
+    if (desc->storage < 0 || desc->storage > 3) return fail(LBFFS_E_INVALID, "storage");
+    if (desc->sgs < 0 || desc->sgs > 2) return fail(LBFFS_E_INVALID, "sgs");
+    if (desc->collision != LBFFS_COLL_RR) return fail(LBFFS_E_UNSUPPORTED, "D3Q19 uses the RR collision");
+    if (desc->block != 0 && (desc->block % 64 != 0 || desc->block > 256 || desc->block < 64))
+        return fail(LBFFS_E_INVALID, "block must be 64, 128, 192 or 256");
+
+    lbffs_ctx *c = new lbffs_ctx();
+    c->d = *desc;
+    c->N = N;
+    c->real_size = (desc->storage == LBFFS_STORE_F64) ? 8 : 4;
+    c->store_size = (desc->storage == LBFFS_STORE_F64) ? 8 : (desc->storage == LBFFS_STORE_F32 ? 4 : 2);
+    const size_t n = (size_t)N, rs = c->real_size;
+    int e = 0;
+    hipError_t he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete c;
+        return fail(LBFFS_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+    }
+    if (!e) e = dalloc(&c->f[0], (size_t)desc->q * n * c->store_size);
+    if (!e) e = dalloc(&c->f[1], (size_t)desc->q * n * c->store_size);
+    if (!e) e = dalloc((void **)&c->boundary, n * 4);
+    if (!e) e = dalloc((void **)&c->b1, n * 4);
+    if (!e) e = dalloc((void **)&c->b2, n * 4);
+    if (!e) e = dalloc((void **)&c->b3, n * 4);
+    if (!e) e = dalloc((void **)&c->solid, n);
+    if (!e) e = dalloc((void **)&c->neiSolid, n);
+    void **reals[] = {&c->sdf, &c->u0, &c->v0, &c->w0, &c->omega, &c->Fwx, &c->Fwy, &c->Fwz, &c->tau, &c->Gx,
+                      &c->Gy, &c->Gz, &c->rho[0], &c->rho[1], &c->u[0], &c->u[1], &c->v[0], &c->v[1], &c->w[0],
+                      &c->w[1]};
+    for (void **p : reals)
+        if (!e) e = dalloc(p, n * rs);
+    if (!e) e = dalloc((void **)&c->flag, n * sizeof(uint16_t));
+    if (!e) e = dalloc((void **)&c->d_err, sizeof(int) * 4);
+    if (!e) e = dalloc((void **)&c->d_part, sizeof(double) * LBFFS_REDUCE_N * kRedBlocks);
+    if (!e) e = dalloc((void **)&c->d_red, sizeof(double) * LBFFS_REDUCE_N);
+    if (!e) {
+        he = hipEventCreate(&c->ev0);
+        if (he == hipSuccess) he = hipEventCreate(&c->ev1);
+        if (he != hipSuccess) e = fail(LBFFS_E_DEVICE, std::string("hipEventCreate: ") + hipGetErrorString(he));
+    }
+    if (e) {
+        free_all(c);
+        delete c;
+        return e;
+    }
+    // sdf defaults to the reference's "no wall" value 10000 (ns.cpp:201-202); omega to omega0
+    if (rs == 4) {
+        std::vector<float> v(n, 10000.f), o(n, desc->omega0);
+        hipMemcpy(c->sdf, v.data(), n * 4, hipMemcpyHostToDevice);
+        hipMemcpy(c->omega, o.data(), n * 4, hipMemcpyHostToDevice);
+    } else {
+        std::vector<double> v(n, 10000.0), o(n, (double)desc->omega0);
+        hipMemcpy(c->sdf, v.data(), n * 8, hipMemcpyHostToDevice);
+        hipMemcpy(c->omega, o.data(), n * 8, hipMemcpyHostToDevice);
+    }
+    he = hipDeviceSynchronize();
+    if (he != hipSuccess) {
+        free_all(c);
+        delete c;
+        return fail(LBFFS_E_DEVICE, std::string("create: ") + hipGetErrorString(he));
+    }
+    *out = c;
+    return 0;
+}
+
+int lbffs_upload(lbffs_ctx *c, int field, const void *host, size_t bytes)
+{
+    if (!c || !host) return fail(LBFFS_E_INVALID, "null argument");
+    if (field < 0 || field >= LBFFS_FIELD_COUNT) return fail(LBFFS_E_INVALID, "field id");
+    if (bytes != field_host_bytes(c, field))
+        return fail(LBFFS_E_INVALID, "upload size " + std::to_string(bytes) + " != " +
+                                         std::to_string(field_host_bytes(c, field)));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (field == LBFFS_FIELD_F) {
+        int e = dispatch<UploadFOp>(c, host);
+        if (!e) c->f_loaded = true;
+        return e;
+    }
+    HIP_TRY(hipMemcpy(field_ptr(c, field), host, bytes, hipMemcpyHostToDevice));
+    if (field_affects_flags(field)) c->flags_dirty = true;
+    return 0;
+}
+
+int lbffs_step(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
+{
+    if (!c || !p) return fail(LBFFS_E_INVALID, "null argument");
+    if (nsteps < 0) return fail(LBFFS_E_INVALID, "nsteps < 0");
+    if (!c->f_loaded) return fail(LBFFS_E_STATE, "populations were never uploaded");
+    if (c->flags_dirty) {
+        int e = is_f64(c) ? pack_flags<double>(c) : pack_flags<float>(c);
+        if (e) return e;
+    }
+    if (nsteps == 0) return 0;
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    int e = dispatch<StepOp>(c, nsteps, p);
+    if (e) return e;
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    c->timed = true;
+    return 0;
+}
+
+int lbffs_download(lbffs_ctx *c, int field, void *host, size_t bytes)
+{
+    if (!c || !host) return fail(LBFFS_E_INVALID, "null argument");
+    if (field < 0 || field >= LBFFS_FIELD_COUNT) return fail(LBFFS_E_INVALID, "field id");
+    if (bytes != field_host_bytes(c, field))
+        return fail(LBFFS_E_INVALID, "download size " + std::to_string(bytes) + " != " +
+                                         std::to_string(field_host_bytes(c, field)));
+    if (field == LBFFS_FIELD_F) return dispatch<DownloadFOp>(c, host);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(host, field_ptr(c, field), bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int lbffs_reduce(lbffs_ctx *c, double out[LBFFS_REDUCE_N])
+{
+    if (!c || !out) return fail(LBFFS_E_INVALID, "null argument");
+    if (c->flags_dirty) {
+        int e = is_f64(c) ? pack_flags<double>(c) : pack_flags<float>(c);
+        if (e) return e;
+    }
+    return dispatch<ReduceOp>(c, out);
+}
+
+int lbffs_synchronize(lbffs_ctx *c)
+{
+    if (!c) return fail(LBFFS_E_INVALID, "null argument");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lbffs_last_step_ms(lbffs_ctx *c, float *ms)
+{
+    if (!c || !ms) return fail(LBFFS_E_INVALID, "null argument");
+    if (!c->timed) return fail(LBFFS_E_STATE, "no step has run");
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    HIP_TRY(hipEventElapsedTime(ms, c->ev0, c->ev1));
+    return 0;
+}
+
+int lbffs_destroy(lbffs_ctx *c)
+{
+    if (!c) return 0;
+    hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return 0;
+}
+
+const char *lbffs_strerror(int code)
+{
+    switch (code) {
+    case LBFFS_OK: return "ok";
+    case LBFFS_E_INVALID: return "invalid argument";
+    case LBFFS_E_NOMEM: return "device out of memory";
+    case LBFFS_E_DEVICE: return "HIP runtime error";
+    case LBFFS_E_UNSUPPORTED: return "unsupported configuration";
+    case LBFFS_E_STATE: return "call out of order";
+    }
+    return "unknown error";
+}
+
+const char *lbffs_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

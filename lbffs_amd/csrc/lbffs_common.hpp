@@ -1,0 +1,151 @@
+// lbffs_common.hpp -- shared device definitions of the MI355X LBM engine:
+// per-cell flag word, 32-bit division by run-time constants, and the storage codecs
+// for populations held in HBM as float, double, or 16-bit deviations from the
+// lattice weights.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+namespace lbffs {
+
+// ---- per-cell flag word (uint16), built once from the reference's side arrays ----
+// bit 0      SOLID      solid cell: no work (k_streamingCollision skips it, kernel.cl:3549)
+// bit 1      FACE       boundary[ic]==1: domain-face streaming + BCs (ns.cpp:164-177)
+// bit 2      NEISOLID   fluid cell with a solid upwind neighbour (walls.hpp:187-216)
+// bit 3      KEEPMAC    later steps read this cell's rho/u/v/w (faces and the layer inside)
+// bit 4      HASSDF     sdf != 10000: Smagorinsky wall damping may apply
+// bit 5      OMEGAVAR   omega differs from the run's omega0: read the omega field
+// bits 6-8   x-face BC code, 9-11 y-face code, 12-14 z-face code (codes 0..7)
+enum : uint32_t {
+    F_SOLID = 1u << 0,
+    F_FACE = 1u << 1,
+    F_NEISOLID = 1u << 2,
+    F_KEEPMAC = 1u << 3,
+    F_HASSDF = 1u << 4,
+    F_OMEGAVAR = 1u << 5,
+    F_B1_SHIFT = 6,
+    F_B2_SHIFT = 9,
+    F_B3_SHIFT = 12,
+};
+
+__host__ __device__ inline int flag_code(uint32_t fl, int shift) { return (int)((fl >> shift) & 7u); }
+
+// BC codes (input.hpp:221-256)
+enum { BC_CYCLIC = 0, BC_BOUNCEBACK = 1, BC_OUTLET = 3, BC_SYMMETRY = 4, BC_FIXEDVELOCITY = 5,
+       BC_FIXEDDENSITY = 6, BC_WALLFUNCTION = 7 };
+
+// a face code that reflects populations instead of wrapping them (kernel.cl:335-337)
+__host__ __device__ inline bool is_wall_code(int c) { return c == 1 || c == 4 || c == 5 || c == 6 || c == 7; }
+
+// ---- unsigned 32-bit division by a run-time constant: q = umulhi(n, m) >> s ----
+// Valid for n < 2^31 (checked by the host: cell counts are below 2^31).
+struct FastDiv {
+    uint32_t d, m, s;
+};
+
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv r{d, 0u, 0u};
+    if (d <= 1) return r;
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;  // ceil(log2 d)
+    uint32_t p = 31 + l;
+    r.m = (uint32_t)(((1ull << p) + d - 1) / d);
+    r.s = p - 32;
+    return r;
+}
+
+__host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f)
+{
+    if (f.d <= 1) return n;
+#ifdef __HIP_DEVICE_COMPILE__
+    return __umulhi(n, f.m) >> f.s;
+#else
+    return (uint32_t)(((uint64_t)n * f.m) >> 32) >> f.s;
+#endif
+}
+
+// ---- storage codecs ----
+// 16-bit forms hold (f_q - w_q) * 2^e_q; the power-of-two scale keeps the stored value
+// O(0.1..1) for every direction and is exact in both directions of the conversion.
+template <int Q>
+struct DevScale;
+
+template <>
+struct DevScale<19> {
+    // w_q = 1/3, 1/18 (axis), 1/36 (diagonal)  ->  scale 4, 16, 32
+    __host__ __device__ static constexpr float scale(int q) { return q == 0 ? 4.0f : (q <= 6 ? 16.0f : 32.0f); }
+    __host__ __device__ static constexpr float inv(int q) { return q == 0 ? 0.25f : (q <= 6 ? 0.0625f : 0.03125f); }
+};
+
+template <>
+struct DevScale<27> {
+    // w = 8/27, 2/27, 1/54, 1/216 -> scale 4, 16, 64, 256
+    __host__ __device__ static constexpr float scale(int q)
+    {
+        return q == 0 ? 4.0f : (q <= 6 ? 16.0f : (q <= 18 ? 64.0f : 256.0f));
+    }
+    __host__ __device__ static constexpr float inv(int q)
+    {
+        return q == 0 ? 0.25f : (q <= 6 ? 0.0625f : (q <= 18 ? 0.015625f : 0.00390625f));
+    }
+};
+
+struct bf16_t {
+    uint16_t bits;
+};
+
+__device__ inline float bf16_to_float(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+
+// round to nearest even; NaN stays NaN
+__device__ inline uint16_t float_to_bf16(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Store<S,Real>: load/store of population q with weight w as compute type Real.
+template <typename S, typename Real, int Q>
+struct Codec;
+
+template <int Q>
+struct Codec<float, float, Q> {
+    __device__ static inline float load(const float *p, int, float) { return *p; }
+    __device__ static inline void store(float *p, float v, int, float) { *p = v; }
+};
+
+template <int Q>
+struct Codec<double, double, Q> {
+    __device__ static inline double load(const double *p, int, double) { return *p; }
+    __device__ static inline void store(double *p, double v, int, double) { *p = v; }
+};
+
+template <int Q>
+struct Codec<__half, float, Q> {
+    __device__ static inline float load(const __half *p, int q, float w)
+    {
+        return w + __half2float(*p) * DevScale<Q>::inv(q);
+    }
+    __device__ static inline void store(__half *p, float v, int q, float w)
+    {
+        *p = __float2half_rn((v - w) * DevScale<Q>::scale(q));
+    }
+};
+
+template <int Q>
+struct Codec<bf16_t, float, Q> {
+    __device__ static inline float load(const bf16_t *p, int q, float w)
+    {
+        return w + bf16_to_float(p->bits) * DevScale<Q>::inv(q);
+    }
+    __device__ static inline void store(bf16_t *p, float v, int q, float w)
+    {
+        p->bits = float_to_bf16((v - w) * DevScale<Q>::scale(q));
+    }
+};
+
+}  // namespace lbffs

@@ -1,0 +1,60 @@
+/*
+ * lbm_oracle.h -- CPU oracle of the reference LBM time step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it.
+ */
+#ifndef LBM_ORACLE_H
+#define LBM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_JACOBI = 0, ORC_LEX_GS = 1 };
+enum { ORC_SGS_WALE = 0, ORC_SGS_SMAGORINSKY = 1, ORC_SGS_NONE = 2 };
+enum { ORC_COLL_RR = 0, ORC_COLL_BGK = 1 };
+
+typedef struct {
+    int nx, ny, nz;
+    int order;     /* ORC_JACOBI | ORC_LEX_GS */
+    int sgs;       /* ORC_SGS_* */
+    int collision; /* ORC_COLL_* */
+    int damping;   /* Smagorinsky wall damping on/off */
+    float dpdx, LES, omegaB, spzWidth, Cs;
+} orc_params;
+
+extern const int ORC_CX[19], ORC_CY[19], ORC_CZ[19];
+
+void orc_fatal(const char *msg);
+long orc_index(int i, int j, int k, int nx, int ny);
+int orc_reflect(int q);
+int orc_reflect_or_mirror(int q, int b1, int b2, int b3);
+long orc_upwind_B(int q, int i, int j, int k, int nx, int ny, int nz, int b1, int b2, int b3);
+long orc_upwind_internal(int q, int i, int j, int k, int nx, int ny);
+long orc_upwind_2D(int q, int i, int j, int k, int nx, int ny, int nz);
+int orc_corner(int b3, int i, int j, int k, int nx, int ny, int nz);
+
+int orc19_step_f32(const orc_params *P, float *f, float *fTmp, const int *boundary, const int *b1, const int *b2,
+                   const int *b3, const float *sdf, const unsigned char *solid, const unsigned char *neiSolid,
+                   const float *u0, const float *v0, const float *w0, float *Fwx, float *Fwy, float *Fwz,
+                   float *tauSGS, float *rho, float *u, float *v, float *w, const float *omega, float *Gx,
+                   float *Gy, float *Gz);
+int orc19_step_f64(const orc_params *P, double *f, double *fTmp, const int *boundary, const int *b1, const int *b2,
+                   const int *b3, const double *sdf, const unsigned char *solid, const unsigned char *neiSolid,
+                   const double *u0, const double *v0, const double *w0, double *Fwx, double *Fwy, double *Fwz,
+                   double *tauSGS, double *rho, double *u, double *v, double *w, const double *omega, double *Gx,
+                   double *Gy, double *Gz);
+void orc19_collide_f32(int kind, const float *ft, float rho, float u, float v, float w, float tau, float tauSGS,
+                       float omegaB, float dpdx, float *out);
+void orc19_collide_f64(int kind, const double *ft, double rho, double u, double v, double w, double tau,
+                       double tauSGS, double omegaB, double dpdx, double *out);
+float orc19_smagorinsky_f32(const float *ft, float rho, float u, float v, float w, float omega, float LES, float sdf,
+                            float Cs, int damping);
+double orc19_smagorinsky_f64(const double *ft, double rho, double u, double v, double w, double omega, double LES,
+                             double sdf, double Cs, int damping);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,187 @@
+"""Python access to the oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads the CPU restatement ``oracle/liboracle.so``, the reference-kernel runner
+``oracle/librefrun.so`` with ``oracle/_ref/kernel19.co`` (the reference's kernel.cl built
+for gfx950), and ``oracle/_ref/libref_host.so`` (the reference's host fragments).  Only
+tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of bench.py use this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REFRUN_SO = os.path.join(HERE, "librefrun.so")
+REF_CO19 = os.path.join(HERE, "_ref", "kernel19.co")
+REF_HOST_SO = os.path.join(HERE, "_ref", "libref_host.so")
+
+JACOBI, LEX_GS = 0, 1
+SGS_WALE, SGS_SMAGORINSKY, SGS_NONE = 0, 1, 2
+COLL_RR, COLL_BGK = 0, 1
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("order", C.c_int), ("sgs", C.c_int),
+                ("collision", C.c_int), ("damping", C.c_int), ("dpdx", C.c_float), ("LES", C.c_float),
+                ("omegaB", C.c_float), ("spzWidth", C.c_float), ("Cs", C.c_float)]
+
+
+_orc = None
+_ref = None
+_refhost = None
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        L = C.CDLL(ORACLE_SO)
+        vp = C.c_void_p
+        for p in ("f32", "f64"):
+            getattr(L, f"orc19_step_{p}").argtypes = [C.POINTER(OrcParams)] + [vp] * 23
+            getattr(L, f"orc19_step_{p}").restype = C.c_int
+        L.orc19_collide_f32.argtypes = [C.c_int, vp] + [C.c_float] * 8 + [vp]
+        L.orc19_collide_f64.argtypes = [C.c_int, vp] + [C.c_double] * 8 + [vp]
+        L.orc19_smagorinsky_f32.argtypes = [vp] + [C.c_float] * 8 + [C.c_int]
+        L.orc19_smagorinsky_f32.restype = C.c_float
+        L.orc19_smagorinsky_f64.argtypes = [vp] + [C.c_double] * 8 + [C.c_int]
+        L.orc19_smagorinsky_f64.restype = C.c_double
+        L.orc_upwind_B.argtypes = [C.c_int] * 10
+        L.orc_upwind_B.restype = C.c_long
+        L.orc_reflect_or_mirror.argtypes = [C.c_int] * 4
+        L.orc_corner.argtypes = [C.c_int] * 7
+        _orc = L
+    return _orc
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+MACRO = ["Fwx", "Fwy", "Fwz", "tauSGS", "rho", "u", "v", "w", "GxIBM", "GyIBM", "GzIBM"]
+
+
+def initial_state(case, real=np.float32):
+    """Arrays of a fresh run in the oracle's precision (macros zero, ns.cpp:81-99)."""
+    a = case.arrays
+    N = case.N
+    st = {k: np.zeros(N, real) for k in MACRO}
+    st["f"] = a["f"].astype(real)
+    return st
+
+
+def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=None, sgs=None):
+    """Run nsteps of the restated reference step.  Returns the state dict (f = current
+    populations after the swaps, plus the macro arrays)."""
+    real = np.float64 if precision == "f64" else np.float32
+    a = case.arrays
+    st = initial_state(case, real) if state is None else {k: v.copy() for k, v in state.items()}
+    p = case.params
+    P = OrcParams(nx=case.nx, ny=case.ny, nz=case.nz, order=order, sgs=case.sgs if sgs is None else sgs,
+                  collision=collision, damping=int(p.get("damping", 1)), dpdx=p.get("dpdx", 0.0),
+                  LES=p.get("LES", 0.0), omegaB=p.get("omegaB", 0.1), spzWidth=p.get("spzWidth", 0.1),
+                  Cs=p.get("Cs", 0.1))
+    sides = {k: np.ascontiguousarray(a[k], real) for k in ("sdf", "u0", "v0", "w0", "omega")}
+    ints = {k: np.ascontiguousarray(a[k], np.int32) for k in ("boundary", "boundary1", "boundary2", "boundary3")}
+    u8 = {k: np.ascontiguousarray(a[k], np.uint8) for k in ("solid", "neiSolid")}
+    f = st["f"]
+    ft = f.copy()  # initialization.hpp: fTmp = f
+    fn = getattr(orc(), f"orc19_step_{precision}")
+    for _ in range(nsteps):
+        rc = fn(C.byref(P), _p(f), _p(ft), _p(ints["boundary"]), _p(ints["boundary1"]), _p(ints["boundary2"]),
+                _p(ints["boundary3"]), _p(sides["sdf"]), _p(u8["solid"]), _p(u8["neiSolid"]), _p(sides["u0"]),
+                _p(sides["v0"]), _p(sides["w0"]), _p(st["Fwx"]), _p(st["Fwy"]), _p(st["Fwz"]), _p(st["tauSGS"]),
+                _p(st["rho"]), _p(st["u"]), _p(st["v"]), _p(st["w"]), _p(sides["omega"]), _p(st["GxIBM"]),
+                _p(st["GyIBM"]), _p(st["GzIBM"]))
+        if rc != 0:
+            raise RuntimeError("oracle step failed")
+        f, ft = ft, f
+    st["f"] = f
+    return st
+
+
+def collide(ft, rho, u, v, w, tau, tauSGS=0.0, omegaB=0.1, dpdx=0.0, kind=COLL_RR, precision="f64"):
+    real = np.float64 if precision == "f64" else np.float32
+    a = np.ascontiguousarray(ft, real)
+    out = np.empty(19, real)
+    getattr(orc(), f"orc19_collide_{precision}")(kind, _p(a), rho, u, v, w, tau, tauSGS, omegaB, dpdx, _p(out))
+    return out
+
+
+def smagorinsky(ft, rho, u, v, w, omega, LES=1.0, sdf=10000.0, Cs=0.1, damping=1, precision="f64"):
+    real = np.float64 if precision == "f64" else np.float32
+    a = np.ascontiguousarray(ft, real)
+    return getattr(orc(), f"orc19_smagorinsky_{precision}")(_p(a), rho, u, v, w, omega, LES, sdf, Cs, damping)
+
+
+# ---- the reference kernel, run on the GPU ----
+
+def refrun():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REFRUN_SO)
+        vp = C.c_void_p
+        L.refrun_step19.argtypes = ([C.c_char_p] + [C.c_int] * 5 + [vp] * 23 + [C.c_float] * 3 + [C.c_int]
+                                    + [C.c_float] * 2)
+        L.refrun_step19.restype = C.c_int
+        L.refrun_error.restype = C.c_char_p
+        _ref = L
+    return _ref
+
+
+def run_reference_gpu(case, nsteps, block=256, rho_av=1.0, LES=None):
+    """Run the reference kernel.cl (compiled for gfx950) for nsteps with its own host loop."""
+    if not os.path.exists(REF_CO19):
+        raise FileNotFoundError(REF_CO19 + " (make -C oracle ref)")
+    a = case.arrays
+    N = case.N
+    st = {k: np.zeros(N, np.float32) for k in MACRO}
+    f = np.ascontiguousarray(a["f"], np.float32).copy()
+    p = case.params
+    typed = [f, np.ascontiguousarray(a["boundary"], np.int32), np.ascontiguousarray(a["boundary1"], np.int32),
+             np.ascontiguousarray(a["boundary2"], np.int32), np.ascontiguousarray(a["boundary3"], np.int32),
+             np.ascontiguousarray(a["sdf"], np.float32), np.ascontiguousarray(a["solid"], np.uint8),
+             np.ascontiguousarray(a["neiSolid"], np.uint8), np.ascontiguousarray(a["u0"], np.float32),
+             np.ascontiguousarray(a["v0"], np.float32), np.ascontiguousarray(a["w0"], np.float32),
+             st["Fwx"], st["Fwy"], st["Fwz"], st["tauSGS"], st["rho"], st["u"], st["v"], st["w"],
+             np.ascontiguousarray(a["omega"], np.float32), st["GxIBM"], st["GyIBM"], st["GzIBM"]]
+    rc = refrun().refrun_step19(REF_CO19.encode(), case.nx, case.ny, case.nz, nsteps, block,
+                                *[_p(t) for t in typed],
+                                p.get("dpdx", 0.0), rho_av, p.get("LES", 0.0) if LES is None else LES, 0,
+                                p.get("omegaB", 0.1), p.get("spzWidth", 0.1))
+    if rc != 0:
+        raise RuntimeError("reference run failed: " + refrun().refrun_error().decode())
+    st["f"] = f
+    return st
+
+
+# ---- the reference host fragments ----
+
+def refhost():
+    global _refhost
+    if _refhost is None:
+        L = C.CDLL(REF_HOST_SO)
+        vp = C.c_void_p
+        L.ref_set_boundary_conditions.argtypes = [C.c_int] * 3 + [vp] * 8
+        L.ref_initialization.argtypes = [C.c_int] * 3 + [C.c_float] * 4 + [vp, vp]
+        _refhost = L
+    return _refhost
+
+
+def ref_set_boundary_conditions(nx, ny, nz, codes, vel):
+    N = nx * ny * nz
+    codes = np.ascontiguousarray(codes, np.int32).reshape(6, 3)
+    vel = np.ascontiguousarray(vel, np.float32).reshape(6, 3)
+    out = [np.zeros(N, np.int32) for _ in range(3)] + [np.zeros(N, np.float32) for _ in range(3)]
+    refhost().ref_set_boundary_conditions(nx, ny, nz, _p(codes), _p(vel), *[_p(o) for o in out])
+    return out
+
+
+def ref_initialization(nx, ny, nz, rho0, u, v, w):
+    N = nx * ny * nz
+    f = np.zeros(19 * N, np.float32)
+    ft = np.zeros(19 * N, np.float32)
+    refhost().ref_initialization(nx, ny, nz, rho0, u, v, w, _p(f), _p(ft))
+    return f, ft

@@ -1,0 +1,129 @@
+// ref_runner.cpp -- runs the reference's own k_streamingCollision (D3Q19_opencl/kernel.cl,
+// built by `make -C oracle ref` into oracle/_ref/kernel19.co) on the GPU with the
+// reference host loop: launch over nx*ny*nz work-items, wait, swap f/fTmp
+// (ns.cpp:509-603, without output and moving walls).
+//
+// TEST INFRASTRUCTURE ONLY: the checker that pins the CPU oracle and the HIP product to
+// the reference kernel on the same inputs.  The reference kernel has no bounds guard
+// (kernel.cl:3548) and a few branches that index outside the lattice; refrun_step19
+// refuses inputs that would reach them and launches exactly N work-items.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+std::string g_err;
+
+int err(const std::string &m)
+{
+    g_err = m;
+    return -1;
+}
+
+#define H(x)                                                                          \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) return err(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct Buf {
+    void *p = nullptr;
+    ~Buf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+}  // namespace
+
+extern "C" {
+
+const char *refrun_error(void) { return g_err.c_str(); }
+
+// Arrays are host arrays in the reference layout.  In/out arrays are overwritten with
+// the state after nsteps: f (19N, current populations after the swaps), Fw*, tauSGS,
+// rho/u/v/w, G*.
+int refrun_step19(const char *co_path, int nx, int ny, int nz, int nsteps, int block, float *f,
+                  const int32_t *boundary, const int32_t *b1, const int32_t *b2, const int32_t *b3, const float *sdf,
+                  const uint8_t *solid, const uint8_t *neiSolid, const float *u0, const float *v0, const float *w0,
+                  float *Fwx, float *Fwy, float *Fwz, float *tauSGS, float *rho, float *u, float *v, float *w,
+                  const float *omega, float *Gx, float *Gy, float *Gz, float dpdx, float rho_av, float LES,
+                  int isReadMovingWalls, float omegaB, float spzWidth)
+{
+    const long N = (long)nx * ny * nz;
+    if (nx < 2 || ny < 2 || nz < 1 || N <= 0 || 19 * N >= (1L << 31)) return err("lattice size");
+    if (block <= 0 || block > 1024 || N % block != 0) return err("N must be a multiple of the work-group size");
+    // inputs that would send the reference kernel outside its buffers
+    for (long ic = 0; ic < N; ic++) {
+        const int i = (int)(ic % ((long)nx * ny) % nx), j = (int)(ic % ((long)nx * ny) / nx),
+                  k = (int)(ic / ((long)nx * ny));
+        if (solid[ic]) continue;
+        const bool face = (i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || (nz > 1 && (k == 0 || k == nz - 1)));
+        if (face && boundary[ic] != 1) return err("fluid face cell without boundary flag");
+        if (boundary[ic] == 1 && neiSolid[ic]) return err("neiSolid on a face cell");
+        if (boundary[ic] == 1 && nz > 1 && k == nz - 1 && b3[ic] == 7) return err("WallFunction on nzMax");
+        if (boundary[ic] == 1 && nz == 1 && (b3[ic] == 3 || b3[ic] == 6 || (b3[ic] == 5 && w0[ic] < 0.f)))
+            return err("2-D lattice with a z-face condition sampling k=1");
+    }
+    hipModule_t mod;
+    H(hipModuleLoad(&mod, co_path));
+    hipFunction_t fn;
+    hipError_t ge = hipModuleGetFunction(&fn, mod, "k_streamingCollision");
+    if (ge != hipSuccess) {
+        (void)hipModuleUnload(mod);
+        return err(std::string("hipModuleGetFunction: ") + hipGetErrorString(ge));
+    }
+    const size_t nf = (size_t)N * sizeof(float), ni = (size_t)N * sizeof(int32_t);
+    Buf F, FT, B, B1, B2, B3, SDF, SOL, NEI, U0, V0, W0, FX, FY, FZ, TAU, RHO, U, V, W, OM, GX, GY, GZ;
+    struct {
+        Buf *b;
+        const void *src;
+        size_t bytes;
+    } up[] = {{&F, f, 19 * nf},     {&FT, f, 19 * nf},   {&B, boundary, ni}, {&B1, b1, ni},     {&B2, b2, ni},
+              {&B3, b3, ni},        {&SDF, sdf, nf},     {&SOL, solid, (size_t)N}, {&NEI, neiSolid, (size_t)N},
+              {&U0, u0, nf},        {&V0, v0, nf},       {&W0, w0, nf},      {&FX, Fwx, nf},    {&FY, Fwy, nf},
+              {&FZ, Fwz, nf},       {&TAU, tauSGS, nf},  {&RHO, rho, nf},    {&U, u, nf},       {&V, v, nf},
+              {&W, w, nf},          {&OM, omega, nf},    {&GX, Gx, nf},      {&GY, Gy, nf},     {&GZ, Gz, nf}};
+    int rc = 0;
+    for (auto &x : up) {
+        if (hipMalloc(&x.b->p, x.bytes) != hipSuccess) {
+            rc = err("hipMalloc");
+            break;
+        }
+        if (hipMemcpy(x.b->p, x.src, x.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = err("hipMemcpy H2D");
+            break;
+        }
+    }
+    if (rc == 0) {
+        unsigned elements = (unsigned)N;
+        void *fsrc = F.p, *fdst = FT.p;
+        for (int s = 0; s < nsteps && rc == 0; s++) {
+            void *params[] = {&fsrc, &fdst, &B.p, &B1.p, &B2.p, &B3.p, &SDF.p, &SOL.p, &NEI.p, &U0.p, &V0.p, &W0.p,
+                              &FX.p, &FY.p, &FZ.p, &TAU.p, &RHO.p, &U.p, &V.p, &W.p, &OM.p, &GX.p, &GY.p, &GZ.p,
+                              &elements, &dpdx, &rho_av, &nx, &ny, &nz, &LES, &isReadMovingWalls, &omegaB,
+                              &spzWidth};
+            hipError_t e = hipModuleLaunchKernel(fn, (unsigned)(N / block), 1, 1, (unsigned)block, 1, 1, 0, nullptr,
+                                                 params, nullptr);
+            if (e == hipSuccess) e = hipDeviceSynchronize();  // queue.finish(), ns.cpp:537
+            if (e != hipSuccess) rc = err(std::string("launch: ") + hipGetErrorString(e));
+            std::swap(fsrc, fdst);  // ns.cpp:602
+        }
+        struct {
+            void *dst;
+            const void *src;
+            size_t bytes;
+        } down[] = {{f, fsrc, 19 * nf}, {Fwx, FX.p, nf}, {Fwy, FY.p, nf}, {Fwz, FZ.p, nf}, {tauSGS, TAU.p, nf},
+                    {rho, RHO.p, nf},   {u, U.p, nf},    {v, V.p, nf},    {w, W.p, nf},    {Gx, GX.p, nf},
+                    {Gy, GY.p, nf},     {Gz, GZ.p, nf}};
+        for (auto &x : down)
+            if (rc == 0 && hipMemcpy(x.dst, x.src, x.bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = err("D2H");
+    }
+    (void)hipModuleUnload(mod);
+    return rc;
+}
+
+}  // extern "C"
