@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Throughput of the fused D3Q19 time step on one MI355X, in the reference's metric.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload backstep|cylinder|cavity]
+
+Default workload: backward-facing step, D3Q19, Smagorinsky LES (Cs=0.1, wall damping),
+1536 x 192 x 128 lattice (37.7M cells), FixedVelocity inlet with seeded 5 % Gaussian
+turbulence, Geier outlet + sponge, Re_H = 5500 -- synthetic inputs built by
+lbffs_amd.cases.backward_step.  One step = one fused stream-collide launch over the
+whole lattice with the populations resident in HBM.
+
+Printed: one JSON line.  value = lattice updates per second of the FP32-storage form
+in MLUPS over all nx*ny*nz cells (the reference's own count, ns.cpp:607); the
+FP16-deviation form is measured in the same run and reported beside it, with the
+velocity error of both forms against an FP64 run of the same steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import lbffs_amd as lb  # noqa: E402
+from lbffs_amd import cases  # noqa: E402
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
+PROFILE_PMC = os.path.join(ROOT, "profiles", "step19_backstep_pmc.txt")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="backstep", choices=["backstep", "cylinder", "cavity"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--only", default="", help="comma list of storage forms to run (f32,f16,bf16)")
+    return ap.parse_args()
+
+
+def build_case(name):
+    if name == "backstep":
+        return cases.backward_step()
+    if name == "cylinder":
+        return cases.cylinder_channel()
+    return cases.cavity(64)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(case, storage, warmup, steps):
+    """Warm up, then time exactly `steps` steps bracketed by device synchronisation."""
+    lat = lb.Lattice(case.nx, case.ny, case.nz, storage=storage, sgs=case.sgs, omega0=case.omega0)
+    case.upload_to(lat)
+    if warmup:
+        lat.step(warmup, **case.params)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    lat.step(steps, **case.params)
+    lat.synchronize()
+    t1 = time.perf_counter()
+    dev_ms = lat.last_step_ms()
+    return lat, (t1 - t0), dev_ms
+
+
+def velocity(lat):
+    return np.concatenate([lat.download(k).astype(np.float64) for k in ("u", "v", "w")])
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def read_pmc():
+    """HBM bytes per launch from the committed counter summary (FETCH_SIZE / WRITE_SIZE
+    of the step kernel, calibrated against the copy probe), or None."""
+    if not os.path.exists(PROFILE_PMC):
+        return None
+    vals = {}
+    with open(PROFILE_PMC) as fh:
+        for line in fh:
+            if "=" in line and not line.lstrip().startswith("#"):
+                k, v = line.split("=", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(case, nsteps):
+    """The oracle (CPU restatement of kernel.cl) on the host cores, same lattice."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po  # noqa: E402
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    po.run19(case, nsteps, order=po.JACOBI)
+    dt = time.perf_counter() - t0
+    return {"value": nsteps * case.N / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} steps of the full {case.nx}x{case.ny}x{case.nz} lattice, oracle FP32 "
+                      f"(Jacobi order, OpenMP {threads} threads), {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    log(f"[bench] building workload {a.workload}")
+    case = build_case(a.workload)
+    N = case.N
+    fluid = int(N - int(case.arrays["solid"].sum()))
+    forms = {"f32": lb.STORE_F32, "f16": lb.STORE_F16DEV, "bf16": lb.STORE_BF16DEV}
+    run = [f for f in (a.only.split(",") if a.only else ["f32", "f16"]) if f]
+    res = {}
+    vel = {}
+    for name in run:
+        log(f"[bench] {name}: warmup {a.warmup}, timed {a.steps}")
+        lat, wall, dev_ms = run_form(case, forms[name], a.warmup, a.steps)
+        bytes_per_cell = 2 * 19 * (4 if name == "f32" else 2)
+        launch_ms = dev_ms / a.steps
+        res[name] = {"wall_s": wall, "dev_ms": dev_ms, "launch_ms": launch_ms,
+                     "mlups": a.steps * N / wall / 1e6, "bytes_per_cell": bytes_per_cell,
+                     "achieved_Bps": fluid * bytes_per_cell / (launch_ms * 1e-3)}
+        if not a.no_error:
+            vel[name] = velocity(lat)
+        lat.close()
+        log(f"[bench] {name}: {res[name]['mlups']:.0f} MLUPS, {launch_ms:.3f} ms/launch, "
+            f"{res[name]['achieved_Bps'] / 1e9:.0f} GB/s")
+    err = {}
+    if not a.no_error and vel:
+        log("[bench] f64 run for the error reference")
+        lat, _, _ = run_form(case, lb.STORE_F64, a.warmup, a.steps)
+        v64 = velocity(lat)
+        lat.close()
+        for k, v in vel.items():
+            err[k] = rel(v, v64)
+    copy_bytes = 4 << 30
+    copy_ms = lb.probe_copy(copy_bytes, reps=10)
+    copy_Bps = 2 * copy_bytes / (copy_ms * 1e-3)
+    main_form = "f32" if "f32" in res else run[0]
+    r = res[main_form]
+    traffic = read_pmc() if a.workload == "backstep" and main_form == "f32" else None
+    out = {
+        "metric": "MLUPS + % of HBM roofline, FP32 vs FP16-storage D3Q19 at stated error",
+        "value": round(r["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(r["wall_s"] / a.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if main_form == "f32" else ("f16-storage/f32-math" if main_form == "f16" else "bf16-storage/f32-math"),
+        "data": "synthetic",
+        "config": {"workload": {"backstep": "backward-facing step D3Q19 Smagorinsky LES Re_H=5500",
+                                "cylinder": "channel past cylinder D3Q19 Re=100",
+                                "cavity": "lid-driven cavity D3Q19 Re=100"}[a.workload],
+                   "lattice": [case.nx, case.ny, case.nz], "cells": N, "fluid_cells": fluid,
+                   "collision": "recursive-regularized",
+                   "sgs": {0: "WALE", 1: "Smagorinsky", 2: "none"}[case.sgs],
+                   "storage": main_form, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved_Bps"] / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(r["achieved_Bps"] / HBM_PEAK, 4),
+                     "traffic": traffic,
+                     "bytes_per_cell_update": r["bytes_per_cell"], "kernel": "step19_kernel",
+                     "launch_ms": round(r["launch_ms"], 4), "copy_probe_GBps": round(copy_Bps / 1e9, 1)},
+        "forms": {k: {"mlups": round(v["mlups"], 1), "launch_ms": round(v["launch_ms"], 4),
+                      "hbm_frac": round(v["achieved_Bps"] / HBM_PEAK, 4),
+                      "velocity_rel_l2_vs_f64": (round(err[k], 8) if k in err else None),
+                      "error_steps": a.warmup + a.steps}
+                  for k, v in res.items()},
+    }
+    if "f16" in res and "f32" in res:
+        out["forms"]["f16_over_f32"] = round(res["f16"]["mlups"] / res["f32"]["mlups"], 3)
+    if not a.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle on host cores)")
+        out["cpu_baseline"] = cpu_baseline(case, 3 if N > 10_000_000 else 20)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

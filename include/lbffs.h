@@ -150,6 +150,11 @@ int lbffs_destroy(lbffs_ctx *ctx);
 const char *lbffs_strerror(int code);
 const char *lbffs_last_error(void);
 
+/* Diagnostic, no counterpart in the reference: time a plain device-to-device copy of
+ * `bytes` (multiple of 16) on the current device, averaged over `reps` passes.  Gives
+ * the practical HBM ceiling and the counter calibration for traffic measurements. */
+int lbffs_probe_copy(size_t bytes, int reps, float *ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,9 +90,11 @@ def lib():
         L.lbffs_case_init_feq.argtypes = [ip, ip, ip, ip, C.c_float, C.c_float, C.c_float, C.c_float, vp]
         L.lbffs_case_neisolid.argtypes = [ip, ip, ip, ip, vp, vp]
         L.lbffs_case_outlet_sponge.argtypes = [ip] * 10 + [C.c_float, vp]
+        L.lbffs_probe_copy.argtypes = [sz, ip, C.POINTER(C.c_float)]
         for n in ("lbffs_create", "lbffs_upload", "lbffs_download", "lbffs_step", "lbffs_reduce",
                   "lbffs_synchronize", "lbffs_last_step_ms", "lbffs_destroy", "lbffs_case_units",
-                  "lbffs_case_faces", "lbffs_case_init_feq", "lbffs_case_neisolid", "lbffs_case_outlet_sponge"):
+                  "lbffs_case_faces", "lbffs_case_init_feq", "lbffs_case_neisolid", "lbffs_case_outlet_sponge",
+                  "lbffs_probe_copy"):
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib
@@ -106,6 +108,13 @@ def _check(rc, what):
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def probe_copy(nbytes, reps=20):
+    """Milliseconds of one device copy of nbytes (lbffs_probe_copy)."""
+    ms = C.c_float()
+    _check(lib().lbffs_probe_copy(int(nbytes), int(reps), C.byref(ms)), "lbffs_probe_copy")
+    return ms.value
 
 
 class Lattice:

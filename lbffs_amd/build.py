@@ -47,9 +47,12 @@ def build(verbose: bool = True) -> str:
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
           "-c", os.path.join(CSRC, "lbffs_case.cpp"), "-o", case_o], verbose)
     tmp = LIB + ".tmp"
-    api_o = os.path.join(BUILD, "lbffs_api.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "lbffs_api.hip"), "-o", api_o], verbose)
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", api_o, case_o, "-o", tmp], verbose)
+    objs = []
+    for src in ("lbffs_api.hip", "lbffs_probe.hip"):
+        o = os.path.join(BUILD, src.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o], verbose)
+        objs.append(o)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, case_o, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 
