@@ -58,10 +58,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(case, storage, warmup, steps):
+def run_form(case, storage, warmup, steps, tune=True):
     """Warm up, then time exactly `steps` steps bracketed by device synchronisation."""
     lat = lb.Lattice(case.nx, case.ny, case.nz, storage=storage, sgs=case.sgs, omega0=case.omega0)
     case.upload_to(lat)
+    lat.tuned = lat.autotune(10, **case.params) if tune else None
     if warmup:
         lat.step(warmup, **case.params)
     lat.synchronize()
@@ -130,7 +131,7 @@ def main():
         lat, wall, dev_ms = run_form(case, forms[name], a.warmup, a.steps)
         bytes_per_cell = 2 * 19 * (4 if name == "f32" else 2)
         launch_ms = dev_ms / a.steps
-        res[name] = {"wall_s": wall, "dev_ms": dev_ms, "launch_ms": launch_ms,
+        res[name] = {"wall_s": wall, "dev_ms": dev_ms, "launch_ms": launch_ms, "launch": lat.tuned,
                      "mlups": a.steps * N / wall / 1e6, "bytes_per_cell": bytes_per_cell,
                      "achieved_Bps": fluid * bytes_per_cell / (launch_ms * 1e-3)}
         if not a.no_error:
@@ -141,7 +142,7 @@ def main():
     err = {}
     if not a.no_error and vel:
         log("[bench] f64 run for the error reference")
-        lat, _, _ = run_form(case, lb.STORE_F64, a.warmup, a.steps)
+        lat, _, _ = run_form(case, lb.STORE_F64, a.warmup, a.steps, tune=False)
         v64 = velocity(lat)
         lat.close()
         for k, v in vel.items():
@@ -149,6 +150,12 @@ def main():
     copy_bytes = 4 << 30
     copy_ms = lb.probe_copy(copy_bytes, reps=10)
     copy_Bps = 2 * copy_bytes / (copy_ms * 1e-3)
+    probes = {}
+    for mode, name, mult in ((1, "copy4", 2), (2, "read", 1), (3, "write", 1)):
+        ms = lb.probe_bandwidth(mode, copy_bytes, reps=10)
+        probes[name] = round(mult * copy_bytes / (ms * 1e-3) / 1e9, 1)
+    probes["copy"] = round(copy_Bps / 1e9, 1)
+    log(f"[bench] HBM probes GB/s: {probes}")
     main_form = "f32" if "f32" in res else run[0]
     r = res[main_form]
     traffic = read_pmc() if a.workload == "backstep" and main_form == "f32" else None
@@ -176,10 +183,11 @@ def main():
                      "unit": "GB/s", "frac": round(r["achieved_Bps"] / HBM_PEAK, 4),
                      "traffic": traffic,
                      "bytes_per_cell_update": r["bytes_per_cell"], "kernel": "step19_kernel",
-                     "launch_ms": round(r["launch_ms"], 4), "copy_probe_GBps": round(copy_Bps / 1e9, 1)},
+                     "launch_ms": round(r["launch_ms"], 4), "probes_GBps": probes},
         "forms": {k: {"mlups": round(v["mlups"], 1), "launch_ms": round(v["launch_ms"], 4),
                       "hbm_frac": round(v["achieved_Bps"] / HBM_PEAK, 4),
                       "velocity_rel_l2_vs_f64": (round(err[k], 8) if k in err else None),
+                      "launch": v["launch"],
                       "error_steps": a.warmup + a.steps}
                   for k, v in res.items()},
     }

@@ -139,6 +139,23 @@ int lbffs_download(lbffs_ctx *ctx, int field, void *host, size_t bytes);
 #define LBFFS_REDUCE_N 8
 int lbffs_reduce(lbffs_ctx *ctx, double out[LBFFS_REDUCE_N]);
 
+/* Launch options of the step (no counterpart in the reference, whose work-group size
+ * is left to the OpenCL runtime, ns.cpp:516). */
+enum {
+    LBFFS_TUNE_NT_STORE = 1, /* write new populations with non-temporal stores */
+    LBFFS_TUNE_NO_XCD = 2    /* keep the hardware work-group order across the 8 XCDs */
+};
+
+/* Autotuner: time every candidate work-group size (64, 128, 256) and launch option on
+ * this lattice for trial_steps steps each, keep the fastest for later lbffs_step calls
+ * and report it.  Populations and every field a step writes are restored afterwards, so
+ * the simulation state is unchanged. */
+int lbffs_autotune(lbffs_ctx *ctx, const lbffs_step_params *params, int trial_steps, int *best_block,
+                   int *best_tune, float *best_ms_per_step);
+
+/* Set the launch shape directly (work-group size multiple of 64 up to 256, LBFFS_TUNE_*). */
+int lbffs_set_launch(lbffs_ctx *ctx, int block, int tune);
+
 /* Wait for queued device work. */
 int lbffs_synchronize(lbffs_ctx *ctx);
 
@@ -150,9 +167,12 @@ int lbffs_destroy(lbffs_ctx *ctx);
 const char *lbffs_strerror(int code);
 const char *lbffs_last_error(void);
 
-/* Diagnostic, no counterpart in the reference: time a plain device-to-device copy of
- * `bytes` (multiple of 16) on the current device, averaged over `reps` passes.  Gives
- * the practical HBM ceiling and the counter calibration for traffic measurements. */
+/* Diagnostics, no counterpart in the reference: time plain streaming kernels over
+ * `bytes` (multiple of 16) on the current device, averaged over `reps` passes.  They give
+ * the practical HBM ceilings and the counter calibration for traffic measurements.
+ * mode 0: grid-stride copy; 1: copy, four 16-byte loads per thread issued before the
+ * stores; 2: read only; 3: write only.  lbffs_probe_copy is mode 0. */
+int lbffs_probe_bandwidth(int mode, size_t bytes, int reps, float *ms_per_pass);
 int lbffs_probe_copy(size_t bytes, int reps, float *ms_per_copy);
 
 #ifdef __cplusplus

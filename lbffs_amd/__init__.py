@@ -91,10 +91,14 @@ def lib():
         L.lbffs_case_neisolid.argtypes = [ip, ip, ip, ip, vp, vp]
         L.lbffs_case_outlet_sponge.argtypes = [ip] * 10 + [C.c_float, vp]
         L.lbffs_probe_copy.argtypes = [sz, ip, C.POINTER(C.c_float)]
+        L.lbffs_probe_bandwidth.argtypes = [ip, sz, ip, C.POINTER(C.c_float)]
+        L.lbffs_autotune.argtypes = [vp, C.POINTER(StepParams), ip, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_float)]
+        L.lbffs_set_launch.argtypes = [vp, ip, ip]
         for n in ("lbffs_create", "lbffs_upload", "lbffs_download", "lbffs_step", "lbffs_reduce",
                   "lbffs_synchronize", "lbffs_last_step_ms", "lbffs_destroy", "lbffs_case_units",
                   "lbffs_case_faces", "lbffs_case_init_feq", "lbffs_case_neisolid", "lbffs_case_outlet_sponge",
-                  "lbffs_probe_copy"):
+                  "lbffs_probe_copy", "lbffs_probe_bandwidth", "lbffs_autotune", "lbffs_set_launch"):
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib
@@ -110,10 +114,22 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _params(dpdx=0.0, LES=0.0, omegaB=0.1, spzWidth=0.1, Cs=0.1, damping=1, store_every=0):
+    return StepParams(dpdx=dpdx, LES=LES, omegaB=omegaB, spzWidth=spzWidth, Cs=Cs, damping=damping,
+                      store_every=store_every)
+
+
 def probe_copy(nbytes, reps=20):
     """Milliseconds of one device copy of nbytes (lbffs_probe_copy)."""
     ms = C.c_float()
     _check(lib().lbffs_probe_copy(int(nbytes), int(reps), C.byref(ms)), "lbffs_probe_copy")
+    return ms.value
+
+
+def probe_bandwidth(mode, nbytes, reps=10):
+    """Milliseconds of one pass of probe `mode` over nbytes (lbffs_probe_bandwidth)."""
+    ms = C.c_float()
+    _check(lib().lbffs_probe_bandwidth(int(mode), int(nbytes), int(reps), C.byref(ms)), "lbffs_probe_bandwidth")
     return ms.value
 
 
@@ -149,10 +165,20 @@ class Lattice:
         _check(lib().lbffs_download(self._h, FIELD_ID[name], _ptr(a), a.nbytes), f"download {name}")
         return a
 
-    def step(self, nsteps, dpdx=0.0, LES=0.0, omegaB=0.1, spzWidth=0.1, Cs=0.1, damping=1, store_every=0):
-        p = StepParams(dpdx=dpdx, LES=LES, omegaB=omegaB, spzWidth=spzWidth, Cs=Cs, damping=damping,
-                       store_every=store_every)
+    def step(self, nsteps, **params):
+        p = _params(**params)
         _check(lib().lbffs_step(self._h, int(nsteps), C.byref(p)), "lbffs_step")
+
+    def autotune(self, trial_steps=10, **params):
+        """Pick the fastest launch shape on this lattice (state is restored)."""
+        p = _params(**params)
+        b, t, ms = C.c_int(), C.c_int(), C.c_float()
+        _check(lib().lbffs_autotune(self._h, C.byref(p), int(trial_steps), C.byref(b), C.byref(t), C.byref(ms)),
+               "lbffs_autotune")
+        return {"block": b.value, "tune": t.value, "ms_per_step": ms.value}
+
+    def set_launch(self, block=256, tune=0):
+        _check(lib().lbffs_set_launch(self._h, int(block), int(tune)), "lbffs_set_launch")
 
     def reduce(self):
         out = (C.c_double * REDUCE_N)()

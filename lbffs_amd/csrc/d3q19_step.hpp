@@ -56,6 +56,14 @@ struct Step19Args {
     int mac_all;      // write rho/u/v/w for every fluid cell
     int aux_all;      // write tauSGS / Fw / G for every fluid cell
     int sponge_mask;  // faces whose centre cell is Outlet or FixedDensity (tauSpongeZone)
+    Real inv_spx, inv_spy, inv_spz;  // 1 / (n * spzWidth) per axis
+    int tune;         // TUNE_* launch options chosen by the autotuner
+};
+
+// launch options (uniform per launch)
+enum : int {
+    TUNE_NT_STORE = 1,  // write the new populations with non-temporal stores
+    TUNE_NO_XCD = 2,    // keep the hardware work-group order (no XCD slab remap)
 };
 
 // XCD-aware block order: the hardware deals work-groups round-robin over the 8 XCDs;
@@ -278,67 +286,93 @@ __device__ inline Real wale_tau(const Step19Args<Real> &a, long ic, int i, int j
     return Real(9) * (Cw * Cw * sd15 / (ss25 + sd125));
 }
 
-// kernel.cl:2133-2201 smagorinskyTauSGS; Pi_neq from the incompressible-form feq
+// reciprocal: one v_rcp_f32 in FP32 (<= 1 ulp), an exact quotient in FP64
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double rcp(double x) { return 1.0 / x; }
+// square root: one v_sqrt_f32 in FP32 (<= 1 ulp), the correctly rounded one in FP64
+__device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline double fsqrt(double x) { return sqrt(x); }
+
+// Density, momentum and the second moments sum_q c_a c_b f_q of the streamed
+// populations; shared by the moments update, the Smagorinsky model and the collision.
 template <typename Real>
-__device__ inline Real smagorinsky_tau(const Real *ft, Real rho, Real u, Real v, Real w, Real omega, Real LES,
-                                       Real sdf, Real Cs, int damping)
+struct Mom19 {
+    Real rho, mx, my, mz;
+    Real sxx, syy, szz, sxy, sxz, syz;
+};
+
+template <typename Real>
+__device__ inline Mom19<Real> moments19(const Real *ft)
 {
-    const Real tau = Real(1) / omega;
-    Real Pxx = 0, Pxy = 0, Pxz = 0, Pyy = 0, Pyz = 0, Pzz = 0;
-    const Real uSqr = u * u + v * v + w * w;
-#pragma unroll
-    for (int q = 0; q < 19; q++) {
-        const Real cx = Real(L19::cx[q]), cy = Real(L19::cy[q]), cz = Real(L19::cz[q]);
-        const Real uDotC = u * cx + v * cy + w * cz;
-        const Real feq = (rho + Real(3) * uDotC + Real(4.5) * uDotC * uDotC - Real(1.5) * uSqr) * w19<Real>(q);
-        const Real d = ft[q] - feq;
-        if (L19::cx[q] != 0) Pxx += d;
-        if (L19::cy[q] != 0) Pyy += d;
-        if (L19::cz[q] != 0) Pzz += d;
-        if (L19::cx[q] * L19::cy[q] != 0) Pxy += Real(L19::cx[q] * L19::cy[q]) * d;
-        if (L19::cx[q] * L19::cz[q] != 0) Pxz += Real(L19::cx[q] * L19::cz[q]) * d;
-        if (L19::cy[q] * L19::cz[q] != 0) Pyz += Real(L19::cy[q] * L19::cz[q]) * d;
-    }
-    const Real PP = sqrt(Pxx * Pxx + Pyy * Pyy + Pzz * Pzz + Real(2) * (Pxy * Pxy + Pxz * Pxz + Pyz * Pyz));
-    Real t = LES * Real(0.5) * (-tau + sqrt(tau * tau + Real(18) * Real(1.4142135623730951) * Cs * Cs * PP / rho));
+    Mom19<Real> m;
+    const Real a12 = ft[1] + ft[2], a34 = ft[3] + ft[4], a56 = ft[5] + ft[6];
+    const Real p78 = ft[7] + ft[8], p910 = ft[9] + ft[10], p1112 = ft[11] + ft[12], p1314 = ft[13] + ft[14];
+    const Real p1516 = ft[15] + ft[16], p1718 = ft[17] + ft[18];
+    m.rho = ft[0] + a12 + a34 + a56 + p78 + p910 + p1112 + p1314 + p1516 + p1718;
+    const Real d78 = ft[7] - ft[8], d910 = ft[9] - ft[10], d1112 = ft[11] - ft[12], d1314 = ft[13] - ft[14];
+    const Real d1516 = ft[15] - ft[16], d1718 = ft[17] - ft[18];
+    m.mx = (ft[1] - ft[2]) + d78 + d910 + d1112 + d1314;
+    m.my = (ft[3] - ft[4]) + d78 - d910 + d1516 + d1718;
+    m.mz = (ft[5] - ft[6]) + d1112 - d1314 + d1516 - d1718;
+    m.sxx = a12 + p78 + p910 + p1112 + p1314;
+    m.syy = a34 + p78 + p910 + p1516 + p1718;
+    m.szz = a56 + p1112 + p1314 + p1516 + p1718;
+    m.sxy = p78 - p910;
+    m.sxz = p1112 - p1314;
+    m.syz = p1516 - p1718;
+    return m;
+}
+
+// kernel.cl:2133-2201 smagorinskyTauSGS.  The reference sums c_a c_b (f_q - feq_q) over q
+// with the incompressible feq (rho + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2) w_q, whose second
+// moment is rho/3 delta_ab + u_a u_b; Pi_neq is formed from the shared moments instead.
+template <typename Real>
+__device__ inline Real smagorinsky_tau(const Mom19<Real> &m, Real u, Real v, Real w, Real tau, Real inv_rho,
+                                       Real LES, Real sdf, Real Cs, int damping)
+{
+    const Real r3 = m.rho * (Real(1) / Real(3));
+    const Real Pxx = m.sxx - r3 - u * u, Pyy = m.syy - r3 - v * v, Pzz = m.szz - r3 - w * w;
+    const Real Pxy = m.sxy - u * v, Pxz = m.sxz - u * w, Pyz = m.syz - v * w;
+    const Real PP = fsqrt(Pxx * Pxx + Pyy * Pyy + Pzz * Pzz + Real(2) * (Pxy * Pxy + Pxz * Pxz + Pyz * Pyz));
+    Real t = LES * Real(0.5) *
+             (-tau + fsqrt(tau * tau + Real(18) * Real(1.4142135623730951) * Cs * Cs * PP * inv_rho));
     if (damping == 1 && sdf != Real(10000) && sdf > Real(0)) {
-        const Real m = fmin(Real(1), Real(0.41) * sdf / Cs);
-        t *= m * m;
+        const Real d = fmin(Real(1), Real(0.41) * sdf / Cs);
+        t *= d * d;
     }
     return t;
 }
 
 // kernel.cl:2444-2503 tauSpongeZone
 template <typename Real>
-__device__ inline Real sponge_tau(const Step19Args<Real> &a, Real omega, Real tauSGS, int i, int j, int k)
+__device__ inline Real sponge_tau(const Step19Args<Real> &a, Real tau, Real tauSGS, int i, int j, int k)
 {
-    Real tau = Real(1) / omega;
     const Real spz = a.spz;
     const int m = a.sponge_mask;
     if (m == 0) return tau;
     const Real one_m = Real(1) - tauSGS;
     if ((m & 1) && Real(i) < Real(a.nx) * spz) {
-        const Real fx = Real(i) / (Real(a.nx) * spz);
+        const Real fx = Real(i) * a.inv_spx;
         tau = (Real(1) - fx) * one_m + fx * tau;
     }
     if ((m & 2) && Real(i) > Real(a.nx) * (Real(1) - spz)) {
-        const Real fx = Real(a.nx - i) / (Real(a.nx) * spz);
+        const Real fx = Real(a.nx - i) * a.inv_spx;
         tau = (Real(1) - fx) * one_m + fx * tau;
     }
     if ((m & 4) && Real(j) < Real(a.ny) * spz) {
-        const Real fx = Real(j) / (Real(a.ny) * spz);
+        const Real fx = Real(j) * a.inv_spy;
         tau = (Real(1) - fx) * one_m + fx * tau;
     }
     if ((m & 8) && Real(j) > Real(a.ny) * (Real(1) - spz)) {
-        const Real fx = Real(a.ny - j) / (Real(a.ny) * spz);
+        const Real fx = Real(a.ny - j) * a.inv_spy;
         tau = (Real(1) - fx) * one_m + fx * tau;
     }
     if ((m & 16) && Real(k) < Real(a.nz) * spz) {
-        const Real fx = Real(k) / (Real(a.nz) * spz);
+        const Real fx = Real(k) * a.inv_spz;
         tau = (Real(1) - fx) * one_m + fx * tau;
     }
     if ((m & 32) && Real(k) > Real(a.nz) * (Real(1) - spz)) {
-        const Real fx = Real(a.nz - k) / (Real(a.nz) * spz);
+        const Real fx = Real(a.nz - k) * a.inv_spz;
         tau = (Real(1) - fx) * one_m + fx * tau;
     }
     return tau;
@@ -348,22 +382,18 @@ __device__ inline Real sponge_tau(const Step19Args<Real> &a, Real omega, Real ta
 // 2nd-order non-equilibrium moments, 3rd/4th order rebuilt recursively), with the
 // body force S_q = rho w_q 3 c_qx dpdx.  u is the force-shifted velocity.
 template <typename Real>
-__device__ inline void collide_rr(Real *out, const Real *ft, Real rho, Real u, Real v, Real w, Real tau,
-                                  Real tauSGS, Real omegaB, Real dpdx)
+__device__ inline void collide_rr(Real *out, const Mom19<Real> &m, Real inv_rho, Real u, Real v, Real w,
+                                  Real omegaEff, Real omegaB, Real dpdx)
 {
     const Real cs2 = Real(1) / Real(3), cs4 = Real(1) / Real(9);
-    const Real omegaEff = Real(1) / (tau + tauSGS);
-    // second-order Hermite moments of ft
-    const Real sxx = ft[1] + ft[2] + ft[7] + ft[8] + ft[9] + ft[10] + ft[11] + ft[12] + ft[13] + ft[14];
-    const Real syy = ft[3] + ft[4] + ft[7] + ft[8] + ft[9] + ft[10] + ft[15] + ft[16] + ft[17] + ft[18];
-    const Real szz = ft[5] + ft[6] + ft[11] + ft[12] + ft[13] + ft[14] + ft[15] + ft[16] + ft[17] + ft[18];
-    const Real invRho = Real(1) / rho;
-    const Real R200 = (sxx - cs2 * rho) * invRho;
-    const Real R020 = (syy - cs2 * rho) * invRho;
-    const Real R002 = (szz - cs2 * rho) * invRho;
-    const Real R110 = (ft[7] + ft[8] - ft[9] - ft[10]) * invRho;
-    const Real R101 = (ft[11] + ft[12] - ft[13] - ft[14]) * invRho;
-    const Real R011 = (ft[15] + ft[16] - ft[17] - ft[18]) * invRho;
+    const Real rho = m.rho;
+    // second-order Hermite moments of ft per unit density
+    const Real R200 = (m.sxx - cs2 * rho) * inv_rho;
+    const Real R020 = (m.syy - cs2 * rho) * inv_rho;
+    const Real R002 = (m.szz - cs2 * rho) * inv_rho;
+    const Real R110 = m.sxy * inv_rho;
+    const Real R101 = m.sxz * inv_rho;
+    const Real R011 = m.syz * inv_rho;
 
     const Real E200 = u * u, E020 = v * v, E002 = w * w, E110 = u * v, E101 = u * w, E011 = v * w;
     const Real N200 = R200 - E200, N020 = R020 - E020, N002 = R002 - E002;
@@ -579,13 +609,13 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
 template <typename S, typename Real, int SGS>
 __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
 {
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     const uint32_t icu = blk * blockDim.x + threadIdx.x;
     if (icu >= a.N) return;
     const uint32_t fl = a.flag[icu];
     if (fl & F_SOLID) return;
     const long ic = icu;
-    const long N = a.N;
+    const uint32_t N = a.N;
     const uint32_t t = fdiv(icu, a.dnx);
     const int i = (int)(icu - t * (uint32_t)a.nx);
     const uint32_t kk = fdiv(t, a.dny);
@@ -597,8 +627,20 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
     const Real omega = (fl & F_OMEGAVAR) ? a.omega[ic] : a.omega0;
     Real ft[19];
     if (!(fl & F_FACE)) {
+        // interior pull: uniform per-direction base (SGPRs) + 32-bit byte offset (VGPR),
+        // so each load is one global_load with an SGPR base
+        // (the offset is formed from the compile-time lattice vector, so the three pulls
+        // that share a y/z shift differ only by an immediate)
+        const uint32_t boff = icu * (uint32_t)sizeof(S);
+        const uint32_t sy = (uint32_t)a.nx * (uint32_t)sizeof(S);
+        const uint32_t sz = (a.nz == 1) ? 0u : (uint32_t)a.nx * (uint32_t)a.ny * (uint32_t)sizeof(S);
 #pragma unroll
-        for (int q = 0; q < 19; q++) ft[q] = Codec<S, Real, 19>::load(fs + (long)q * N + (ic - a.off[q]), q, w19<Real>(q));
+        for (int q = 0; q < 19; q++) {
+            const char *fq = (const char *)(fs + (size_t)q * N);
+            const uint32_t yz = boff - (uint32_t)L19::cy[q] * sy - (uint32_t)L19::cz[q] * sz;
+            const uint32_t src = yz - (uint32_t)(L19::cx[q] * (int)sizeof(S));
+            ft[q] = Codec<S, Real, 19>::load((const S *)(fq + src), q, w19<Real>(q));
+        }
     } else {
         face_cell<S, Real>(a, ft, fl, ic, i, j, k, omega);
     }
@@ -612,7 +654,7 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
             const long src = (fl & F_FACE) ? face_upwind(q, i, j, k, a.nx, a.ny, a.nz, b1, b2, b3) : ic - a.off[q];
             if (src >= 0 && (a.flag[src] & F_SOLID)) {
                 const int qb = L19::opp(q);
-                const Real fb = Codec<S, Real, 19>::load(fs + (long)qb * N + ic, qb, w19<Real>(qb));
+                const Real fb = Codec<S, Real, 19>::load(fs + (size_t)qb * N + icu, qb, w19<Real>(qb));
                 ft[q] = fb;
                 Fx += -(fb + fb) * Real(L19::cx[q]);
                 Fy += -(fb + fb) * Real(L19::cy[q]);
@@ -621,19 +663,14 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
         }
     }
 
-    // updateRhoUVW, kernel.cl:3509-3523
-    Real rho = 0, mx = 0, my = 0, mz = 0;
-#pragma unroll
-    for (int q = 0; q < 19; q++) {
-        rho += ft[q];
-        if (L19::cx[q]) mx += Real(L19::cx[q]) * ft[q];
-        if (L19::cy[q]) my += Real(L19::cy[q]) * ft[q];
-        if (L19::cz[q]) mz += Real(L19::cz[q]) * ft[q];
-    }
-    const Real u = mx / rho + Real(0.5) * a.dpdx / rho;
-    const Real v = my / rho, w = mz / rho;
-    const bool mac = a.mac_all || (fl & F_KEEPMAC);
-    if (mac) {
+    // updateRhoUVW, kernel.cl:3509-3523 (u carries the half-step force shift)
+    const Mom19<Real> m = moments19<Real>(ft);
+    const Real rho = m.rho;
+    const Real inv_rho = rcp(rho);
+    const Real hf = Real(0.5) * a.dpdx * inv_rho;
+    const Real u = m.mx * inv_rho + hf;
+    const Real v = m.my * inv_rho, w = m.mz * inv_rho;
+    if (a.mac_all || (fl & F_KEEPMAC)) {
         a.rho_n[ic] = rho;
         a.u_n[ic] = u;
         a.v_n[ic] = v;
@@ -650,22 +687,32 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
         a.Gz[ic] = Real(0);
     }
 
+    const Real tau0 = rcp(omega);
     Real tauSGS;
     if constexpr (SGS == 0) {
         tauSGS = wale_tau<Real>(a, ic, i, j, k, u, v, w);
     } else if constexpr (SGS == 1) {
         const Real sdf = (fl & F_HASSDF) ? a.sdf[ic] : Real(10000);
-        tauSGS = smagorinsky_tau<Real>(ft, rho, u, v, w, omega, a.LES, sdf, a.Cs, a.damping);
+        tauSGS = smagorinsky_tau<Real>(m, u, v, w, tau0, inv_rho, a.LES, sdf, a.Cs, a.damping);
     } else {
         tauSGS = Real(0);
     }
     if (a.aux_all) a.tau[ic] = tauSGS;
 
-    const Real tau = sponge_tau<Real>(a, omega, tauSGS, i, j, k);
+    const Real tau = sponge_tau<Real>(a, tau0, tauSGS, i, j, k);
+    const Real omegaEff = rcp(tau + tauSGS);
     Real out[19];
-    collide_rr<Real>(out, ft, rho, u - Real(0.5) * a.dpdx / rho, v, w, tau, tauSGS, a.omegaB, a.dpdx);
+    collide_rr<Real>(out, m, inv_rho, u - hf, v, w, omegaEff, a.omegaB, a.dpdx);
+    const uint32_t doff = icu * (uint32_t)sizeof(S);
+    if (a.tune & TUNE_NT_STORE) {
 #pragma unroll
-    for (int q = 0; q < 19; q++) Codec<S, Real, 19>::store(fd + (long)q * N + ic, out[q], q, w19<Real>(q));
+        for (int q = 0; q < 19; q++)
+            Codec<S, Real, 19>::store_nt((S *)((char *)(fd + (size_t)q * N) + doff), out[q], q, w19<Real>(q));
+    } else {
+#pragma unroll
+        for (int q = 0; q < 19; q++)
+            Codec<S, Real, 19>::store((S *)((char *)(fd + (size_t)q * N) + doff), out[q], q, w19<Real>(q));
+    }
 }
 
 }  // namespace lbffs

@@ -67,6 +67,8 @@ struct lbffs_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
+    int block = 256;  // work-group size of the step launch
+    int tune = 0;     // TUNE_* launch options
 };
 
 namespace {
@@ -348,6 +350,11 @@ void set_args(lbffs_ctx *c, Step19Args<Real> &a, const lbffs_step_params *p)
     a.omega0 = (Real)c->d.omega0;
     a.damping = p->damping;
     a.sponge_mask = c->sponge_mask;
+    const double spz = p->spzWidth;
+    a.inv_spx = (Real)(spz > 0 ? 1.0 / ((double)nx * spz) : 0.0);
+    a.inv_spy = (Real)(spz > 0 ? 1.0 / ((double)ny * spz) : 0.0);
+    a.inv_spz = (Real)(spz > 0 ? 1.0 / ((double)nz * spz) : 0.0);
+    a.tune = c->tune;
 }
 
 template <typename S, typename Real>
@@ -355,7 +362,7 @@ int run_steps(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
 {
     Step19Args<Real> a;
     set_args<S, Real>(c, a, p);
-    const int block = c->d.block > 0 ? c->d.block : 256;
+    const int block = c->block;
     const unsigned grid = grid_for(c->N, block);
     for (int s = 0; s < nsteps; s++) {
         const bool last = (s == nsteps - 1);
@@ -513,6 +520,7 @@ int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out)
     lbffs_ctx *c = new lbffs_ctx();
     c->d = *desc;
     c->N = N;
+    c->block = desc->block > 0 ? desc->block : 256;
     c->real_size = (desc->storage == LBFFS_STORE_F64) ? 8 : 4;
     c->store_size = (desc->storage == LBFFS_STORE_F64) ? 8 : (desc->storage == LBFFS_STORE_F32 ? 4 : 2);
     const size_t n = (size_t)N, rs = c->real_size;
@@ -602,6 +610,97 @@ int lbffs_step(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
     if (e) return e;
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     c->timed = true;
+    return 0;
+}
+
+int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, int *best_block, int *best_tune,
+                   float *best_ms)
+{
+    if (!c || !p) return fail(LBFFS_E_INVALID, "null argument");
+    if (trial_steps < 1) return fail(LBFFS_E_INVALID, "trial_steps < 1");
+    if (!c->f_loaded) return fail(LBFFS_E_STATE, "populations were never uploaded");
+    if (c->flags_dirty) {
+        int e = is_f64(c) ? pack_flags<double>(c) : pack_flags<float>(c);
+        if (e) return e;
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // save everything a step writes, so that tuning leaves the lattice state unchanged
+    const size_t n = (size_t)c->N, rs = c->real_size;
+    const size_t fbytes = (size_t)c->d.q * n * c->store_size;
+    void *saved_f = nullptr, *saved_r = nullptr;
+    if (int e = dalloc(&saved_f, fbytes)) return e;
+    if (int e = dalloc(&saved_r, 11 * n * rs)) {
+        (void)hipFree(saved_f);
+        return e;
+    }
+    void *keep[11] = {c->rho[c->mcur], c->u[c->mcur], c->v[c->mcur], c->w[c->mcur], c->tau,
+                      c->Fwx, c->Fwy, c->Fwz, c->Gx, c->Gy, c->Gz};
+    const int cur0 = c->cur, mcur0 = c->mcur, block0 = c->block, tune0 = c->tune;
+    int rc = 0;
+    hipError_t he = hipMemcpyAsync(saved_f, c->f[cur0], fbytes, hipMemcpyDeviceToDevice, c->stream);
+    for (int k = 0; k < 11 && he == hipSuccess; k++)
+        he = hipMemcpyAsync((char *)saved_r + k * n * rs, keep[k], n * rs, hipMemcpyDeviceToDevice, c->stream);
+    const int blocks[3] = {64, 128, 256};
+    const int tunes[4] = {0, TUNE_NT_STORE, TUNE_NO_XCD, TUNE_NT_STORE | TUNE_NO_XCD};
+    float bms = 1e30f;
+    int bb = block0, bt = tune0;
+    for (int bi = 0; bi < 3 && he == hipSuccess && rc == 0; bi++) {
+        for (int ti = 0; ti < 4 && he == hipSuccess && rc == 0; ti++) {
+            c->block = blocks[bi];
+            c->tune = tunes[ti];
+            rc = dispatch<StepOp>(c, 1, p);  // warm the shape
+            if (rc) break;
+            he = hipEventRecord(c->ev0, c->stream);
+            if (he != hipSuccess) break;
+            rc = dispatch<StepOp>(c, trial_steps, p);
+            if (rc) break;
+            he = hipEventRecord(c->ev1, c->stream);
+            if (he == hipSuccess) he = hipEventSynchronize(c->ev1);
+            float ms = 0.f;
+            if (he == hipSuccess) he = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+            ms /= trial_steps;
+            if (he == hipSuccess && ms < bms) {
+                bms = ms;
+                bb = blocks[bi];
+                bt = tunes[ti];
+            }
+        }
+    }
+    // restore the state
+    c->cur = cur0;
+    c->mcur = mcur0;
+    if (he == hipSuccess) he = hipMemcpyAsync(c->f[cur0], saved_f, fbytes, hipMemcpyDeviceToDevice, c->stream);
+    for (int k = 0; k < 11 && he == hipSuccess; k++)
+        he = hipMemcpyAsync(keep[k], (char *)saved_r + k * n * rs, n * rs, hipMemcpyDeviceToDevice, c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    (void)hipFree(saved_f);
+    (void)hipFree(saved_r);
+    if (rc) {
+        c->block = block0;
+        c->tune = tune0;
+        return rc;
+    }
+    if (he != hipSuccess) {
+        c->block = block0;
+        c->tune = tune0;
+        return fail(LBFFS_E_DEVICE, std::string("autotune: ") + hipGetErrorString(he));
+    }
+    c->block = bb;
+    c->tune = bt;
+    c->timed = false;
+    if (best_block) *best_block = bb;
+    if (best_tune) *best_tune = bt;
+    if (best_ms) *best_ms = bms;
+    return 0;
+}
+
+int lbffs_set_launch(lbffs_ctx *c, int block, int tune)
+{
+    if (!c) return fail(LBFFS_E_INVALID, "null argument");
+    if (block % 64 != 0 || block < 64 || block > 256) return fail(LBFFS_E_INVALID, "block must be 64, 128, 192 or 256");
+    if (tune & ~(TUNE_NT_STORE | TUNE_NO_XCD)) return fail(LBFFS_E_INVALID, "unknown launch option");
+    c->block = block;
+    c->tune = tune;
     return 0;
 }
 

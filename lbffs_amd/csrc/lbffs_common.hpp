@@ -116,12 +116,14 @@ template <int Q>
 struct Codec<float, float, Q> {
     __device__ static inline float load(const float *p, int, float) { return *p; }
     __device__ static inline void store(float *p, float v, int, float) { *p = v; }
+    __device__ static inline void store_nt(float *p, float v, int, float) { __builtin_nontemporal_store(v, p); }
 };
 
 template <int Q>
 struct Codec<double, double, Q> {
     __device__ static inline double load(const double *p, int, double) { return *p; }
     __device__ static inline void store(double *p, double v, int, double) { *p = v; }
+    __device__ static inline void store_nt(double *p, double v, int, double) { __builtin_nontemporal_store(v, p); }
 };
 
 template <int Q>
@@ -134,6 +136,11 @@ struct Codec<__half, float, Q> {
     {
         *p = __float2half_rn((v - w) * DevScale<Q>::scale(q));
     }
+    __device__ static inline void store_nt(__half *p, float v, int q, float w)
+    {
+        __half h = __float2half_rn((v - w) * DevScale<Q>::scale(q));
+        __builtin_nontemporal_store(__half_as_ushort(h), (unsigned short *)p);
+    }
 };
 
 template <int Q>
@@ -145,6 +152,10 @@ struct Codec<bf16_t, float, Q> {
     __device__ static inline void store(bf16_t *p, float v, int q, float w)
     {
         p->bits = float_to_bf16((v - w) * DevScale<Q>::scale(q));
+    }
+    __device__ static inline void store_nt(bf16_t *p, float v, int q, float w)
+    {
+        __builtin_nontemporal_store(float_to_bf16((v - w) * DevScale<Q>::scale(q)), (unsigned short *)p);
     }
 };
 

@@ -20,10 +20,14 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def run_hip(case, nsteps, storage=lb.STORE_F32, store_every=0, block=0, chunks=1):
+def run_hip(case, nsteps, storage=lb.STORE_F32, store_every=0, block=0, chunks=1, tune=None, autotune=False):
     with lb.Lattice(case.nx, case.ny, case.nz, storage=storage, sgs=case.sgs, omega0=case.omega0,
                     block=block) as lat:
         case.upload_to(lat)
+        if tune is not None:
+            lat.set_launch(block or 256, tune)
+        if autotune:
+            lat.autotune(3, **case.params)
         left = nsteps
         for c in range(chunks):
             n = left // (chunks - c)
@@ -150,6 +154,35 @@ def test_block_size_invariance(block):
     ref = run_hip(c, 30)
     h = run_hip(c, 30, block=block)
     assert np.array_equal(ref["f"], h["f"])
+
+
+@pytest.mark.parametrize("tune", [1, 2, 3])
+def test_launch_options_bitwise_identical(tune):
+    # non-temporal stores and the work-group order change no arithmetic
+    c = cases.cylinder_channel(nx=64, ny=32, nz=4, D=8, cx=20, cy=16, spz_cells=6, sgs=lb.SGS_SMAGORINSKY)
+    a = run_hip(c, 25)
+    b = run_hip(c, 25, tune=tune)
+    assert np.array_equal(a["f"], b["f"])
+
+
+def test_autotune_restores_state():
+    c = cases.cylinder_channel(nx=64, ny=32, nz=4, D=8, cx=20, cy=16, spz_cells=6, sgs=lb.SGS_SMAGORINSKY)
+    a = run_hip(c, 25)
+    b = run_hip(c, 25, autotune=True)
+    for k in ("f", "rho", "u", "tauSGS", "Fwx"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("storage", [lb.STORE_F16DEV, lb.STORE_BF16DEV])
+def test_16bit_storage_accuracy(storage):
+    # 16-bit deviation storage with FP32 arithmetic: velocity within the stated 1e-3
+    # relative L2 of the FP64 run after a few hundred steps of a developing flow
+    c = cases.cylinder_channel(nx=96, ny=48, nz=8, D=12, cx=30, cy=24, Re=60.0, spz_cells=8, sgs=lb.SGS_SMAGORINSKY)
+    h = run_hip(c, 300, storage=storage)
+    d = run_hip(c, 300, storage=lb.STORE_F64)
+    e = rel_l2(_vel(h), _vel(d))
+    print("16-bit storage", storage, f"velocity rel L2 vs f64 {e:.2e}", flush=True)
+    assert e <= (1e-3 if storage == lb.STORE_F16DEV else 1e-2)
 
 
 def test_f64_vs_oracle_f64():
