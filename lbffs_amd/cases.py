@@ -128,6 +128,20 @@ def poiseuille(nx=100, ny=20, nz=1, dpdx=0.08, nu=0.01, Lx=5.0, U0=0.05, sgs=SGS
     return make_case("poiseuille", nx, ny, nz, Lx, 1.0, U0, nu, codes, dpdx=dpdx, sgs=sgs)
 
 
+def channel(nx=8, ny=20, nz=1, tau=0.8, force=1e-5, sgs=SGS_NONE):
+    """Force-driven periodic channel in lattice units: half-way bounce-back walls in y,
+    body force `force` along x (inverse of the conversions of ns.cpp:104-112)."""
+    U0 = 0.1
+    L = 1.0 / (nx - 1)
+    c = 1.0 / U0
+    nu = (tau - 0.5) / 3.0 * L * c
+    dpdx = force * c * c / L
+    codes = {f: ("Cyclic", "Cyclic", "Cyclic") for f in FACES}
+    codes["nyMin"] = ("Cyclic", "BounceBack", "Cyclic")
+    codes["nyMax"] = ("Cyclic", "BounceBack", "Cyclic")
+    return make_case("channel", nx, ny, nz, 1.0, 1.0, U0, nu, codes, dpdx=dpdx, sgs=sgs)
+
+
 def cylinder_channel(nx=400, ny=160, nz=16, D=40, cx=100, cy=80, u_in=0.05, noise=1e-3, Re=100.0,
                      spz_cells=20, sgs=SGS_WALE, seed=SEED):
     """Channel flow past a circular cylinder: FixedVelocity inlet (u_in plus seeded uniform

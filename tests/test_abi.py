@@ -1,0 +1,53 @@
+"""The C-ABI library: it loads without a GPU, exports every entry point that
+include/*.h declares, and reports errors as codes (no compute calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import lbffs_amd as lb
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for h in ("lbffs.h", "lbffs_case.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(?:int|const char \*)\s*(lbffs_\w+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = lb.lib()
+    names = declared()
+    assert len(names) >= 15
+    for n in sorted(names):
+        assert hasattr(L, n), n
+
+
+def test_strerror_and_invalid_arguments():
+    L = lb.lib()
+    assert L.lbffs_strerror(0) == b"ok"
+    assert L.lbffs_strerror(-1) == b"invalid argument"
+    assert L.lbffs_create(None, None) == -1
+    assert b"null" in L.lbffs_last_error()
+    d = lb.Desc(abi_version=99, q=19, nx=4, ny=4, nz=4)
+    h = ctypes.c_void_p()
+    assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    d = lb.Desc(abi_version=lb.ABI_VERSION, q=19, nx=1, ny=4, nz=4)
+    assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    d = lb.Desc(abi_version=lb.ABI_VERSION, q=19, nx=4, ny=4, nz=4, storage=9)
+    assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert L.lbffs_upload(None, 0, None, 0) == -1
+    assert L.lbffs_step(None, 1, None) == -1
+    assert L.lbffs_destroy(None) == 0
+
+
+def test_missing_library_raises(monkeypatch):
+    monkeypatch.setattr(lb, "_lib", None)
+    monkeypatch.setattr(lb, "LIB_PATH", "/nonexistent/liblbffs.so")
+    with pytest.raises(lb.LbffsError):
+        lb.lib()
