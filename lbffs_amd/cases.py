@@ -33,6 +33,7 @@ class Case:
     params: dict = field(default_factory=dict)  # lbffs_step_params scalars
     units: dict = field(default_factory=dict)
     meta: dict = field(default_factory=dict)
+    q: int = 19
 
     @property
     def N(self):
@@ -90,11 +91,13 @@ def make_case(name, nx, ny, nz, Lx, uMax, U0, nu, face_codes, face_vels=None, rh
                   u0=u0, v0=v0, w0=w0, omega=np.full(N, un.omega0, np.float32))
     p = dict(dpdx=float(un.dpdx), LES=0.0, omegaB=0.1, spzWidth=0.1, Cs=0.1, damping=1)
     p.update(params)
+    if q == 27:
+        p.setdefault("clim", 0.01)
     uu = {k: getattr(un, k) for k in ("L", "c", "Re", "nu", "dpdx", "omega0", "deltaT")}
-    return Case(name, nx, ny, nz, arrays, float(un.omega0), sgs, p, uu)
+    return Case(name, nx, ny, nz, arrays, float(un.omega0), sgs, p, uu, q=q)
 
 
-def cavity(n=64, nz=None, u_lid=0.1, Re=100.0, sgs=SGS_WALE):
+def cavity(n=64, nz=None, u_lid=0.1, Re=100.0, sgs=SGS_WALE, q=19):
     """Lid-driven cavity: lid = nyMax FixedVelocity along +x, other faces half-way
     bounce-back (testCases/cavityFlow/boundaryConditions.txt:23-28).  Physical inputs are
     chosen so that the lattice lid speed is u_lid and nu = u_lid*n/Re (Lx=1, uMax=1)."""
@@ -107,17 +110,17 @@ def cavity(n=64, nz=None, u_lid=0.1, Re=100.0, sgs=SGS_WALE):
     if nz == 1:
         codes["nzMin"] = codes["nzMax"] = ("Cyclic", "Cyclic", "Cyclic")
     vels = {"nyMax": (1.0, 0.0, 0.0)}
-    return make_case(f"cavity{n}", n, n, nz, 1.0, 1.0, U0, nu, codes, vels, sgs=sgs)
+    return make_case(f"cavity{n}", n, n, nz, 1.0, 1.0, U0, nu, codes, vels, sgs=sgs, q=q)
 
 
-def periodic_box(nx, ny, nz, tau=0.8, ini=(0.0, 0.0, 0.0), sgs=SGS_NONE):
+def periodic_box(nx, ny, nz, tau=0.8, ini=(0.0, 0.0, 0.0), sgs=SGS_NONE, q=19):
     """All faces Cyclic: a fully periodic lattice (faces still take the face path)."""
     nu_lat = (tau - 0.5) / 3.0
     U0 = 0.1
     nu = nu_lat * (1.0 / (nx - 1)) * (1.0 / U0)
     # lattice velocity = physical / c with c = uMax/U0 (ns.cpp:105, 113-115)
     return make_case(f"box{nx}x{ny}x{nz}", nx, ny, nz, 1.0, 1.0, U0, nu, {}, ini=tuple(x / U0 for x in ini),
-                     sgs=sgs)
+                     sgs=sgs, q=q)
 
 
 def poiseuille(nx=100, ny=20, nz=1, dpdx=0.08, nu=0.01, Lx=5.0, U0=0.05, sgs=SGS_NONE):
@@ -128,7 +131,7 @@ def poiseuille(nx=100, ny=20, nz=1, dpdx=0.08, nu=0.01, Lx=5.0, U0=0.05, sgs=SGS
     return make_case("poiseuille", nx, ny, nz, Lx, 1.0, U0, nu, codes, dpdx=dpdx, sgs=sgs)
 
 
-def channel(nx=8, ny=20, nz=1, tau=0.8, force=1e-5, sgs=SGS_NONE):
+def channel(nx=8, ny=20, nz=1, tau=0.8, force=1e-5, sgs=SGS_NONE, q=19):
     """Force-driven periodic channel in lattice units: half-way bounce-back walls in y,
     body force `force` along x (inverse of the conversions of ns.cpp:104-112)."""
     U0 = 0.1
@@ -139,11 +142,11 @@ def channel(nx=8, ny=20, nz=1, tau=0.8, force=1e-5, sgs=SGS_NONE):
     codes = {f: ("Cyclic", "Cyclic", "Cyclic") for f in FACES}
     codes["nyMin"] = ("Cyclic", "BounceBack", "Cyclic")
     codes["nyMax"] = ("Cyclic", "BounceBack", "Cyclic")
-    return make_case("channel", nx, ny, nz, 1.0, 1.0, U0, nu, codes, dpdx=dpdx, sgs=sgs)
+    return make_case("channel", nx, ny, nz, 1.0, 1.0, U0, nu, codes, dpdx=dpdx, sgs=sgs, q=q)
 
 
 def cylinder_channel(nx=400, ny=160, nz=16, D=40, cx=100, cy=80, u_in=0.05, noise=1e-3, Re=100.0,
-                     spz_cells=20, sgs=SGS_WALE, seed=SEED):
+                     spz_cells=20, sgs=SGS_WALE, seed=SEED, q=19, sdf_from_geometry=False):
     """Channel flow past a circular cylinder: FixedVelocity inlet (u_in plus seeded uniform
     noise per inlet cell), Outlet (Geier convective) at nxMax with a kernel sponge of
     spz_cells, bounce-back walls in y, periodic z; cylinder cells (i-cx)^2+(j-cy)^2 < (D/2)^2
@@ -160,7 +163,12 @@ def cylinder_channel(nx=400, ny=160, nz=16, D=40, cx=100, cy=80, u_in=0.05, nois
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     solid = (((i - cx) ** 2 + (j - cy) ** 2) < (D / 2.0) ** 2).astype(np.uint8).reshape(-1)
     c = make_case("cylinder", nx, ny, nz, 1.0, 1.0, U0, nu, codes, vels, solid=solid, sgs=sgs,
-                  spzWidth=float(spz_cells) / nx)
+                  spzWidth=float(spz_cells) / nx, q=q)
+    if sdf_from_geometry:
+        # signed distance to the circle (negative inside), used by the interpolated bounce-back
+        d = (np.sqrt((i - cx) ** 2 + (j - cy) ** 2) - D / 2.0).reshape(-1)
+        near = np.abs(d) < 3.0
+        c.arrays["sdf"][near] = d[near].astype(np.float32)
     rng = np.random.default_rng(seed)
     inlet = (i.reshape(-1) == 0) & (c.arrays["boundary1"] == BC["FixedVelocity"])
     u0 = c.arrays["u0"]

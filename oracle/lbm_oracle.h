@@ -22,6 +22,7 @@ typedef struct {
     int collision; /* ORC_COLL_* */
     int damping;   /* Smagorinsky wall damping on/off */
     float dpdx, LES, omegaB, spzWidth, Cs;
+    float clim;    /* D3Q27 cumulant limiter (input.hpp default 0.01) */
 } orc_params;
 
 extern const int ORC_CX[19], ORC_CY[19], ORC_CZ[19];
@@ -53,6 +54,26 @@ float orc19_smagorinsky_f32(const float *ft, float rho, float u, float v, float 
                             float Cs, int damping);
 double orc19_smagorinsky_f64(const double *ft, double rho, double u, double v, double w, double omega, double LES,
                              double sdf, double Cs, int damping);
+
+/* D3Q27 */
+extern const int ORC27_CX[27], ORC27_CY[27], ORC27_CZ[27];
+int orc27_reflect(int q);
+int orc27_mirror(int q, int axis);
+long orc27_upwind_B(int q, int i, int j, int k, int nx, int ny, int nz, int b1, int b2, int b3);
+int orc27_step_f32(const orc_params *P, float *f, float *fTmp, const int *boundary, const int *b1, const int *b2,
+                   const int *b3, const float *sdf, const unsigned char *solid, const unsigned char *neiSolid,
+                   const float *u0, const float *v0, const float *w0, float *Fwx, float *Fwy, float *Fwz,
+                   float *tauSGS, float *rho, float *u, float *v, float *w, const float *omega, float *Gx,
+                   float *Gy, float *Gz);
+int orc27_step_f64(const orc_params *P, double *f, double *fTmp, const int *boundary, const int *b1, const int *b2,
+                   const int *b3, const double *sdf, const unsigned char *solid, const unsigned char *neiSolid,
+                   const double *u0, const double *v0, const double *w0, double *Fwx, double *Fwy, double *Fwz,
+                   double *tauSGS, double *rho, double *u, double *v, double *w, const double *omega, double *Gx,
+                   double *Gy, double *Gz);
+void orc27_collide_f32(const float *ft, float rho, float u, float v, float w, float tau, float tauSGS, float omegaB,
+                       float clim, float *out);
+void orc27_collide_f64(const double *ft, double rho, double u, double v, double w, double tau, double tauSGS,
+                       double omegaB, double clim, double *out);
 
 #ifdef __cplusplus
 }

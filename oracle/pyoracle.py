@@ -26,7 +26,7 @@ COLL_RR, COLL_BGK = 0, 1
 class OrcParams(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("order", C.c_int), ("sgs", C.c_int),
                 ("collision", C.c_int), ("damping", C.c_int), ("dpdx", C.c_float), ("LES", C.c_float),
-                ("omegaB", C.c_float), ("spzWidth", C.c_float), ("Cs", C.c_float)]
+                ("omegaB", C.c_float), ("spzWidth", C.c_float), ("Cs", C.c_float), ("clim", C.c_float)]
 
 
 _orc = None
@@ -40,8 +40,14 @@ def orc():
         L = C.CDLL(ORACLE_SO)
         vp = C.c_void_p
         for p in ("f32", "f64"):
-            getattr(L, f"orc19_step_{p}").argtypes = [C.POINTER(OrcParams)] + [vp] * 23
-            getattr(L, f"orc19_step_{p}").restype = C.c_int
+            for lat in ("19", "27"):
+                getattr(L, f"orc{lat}_step_{p}").argtypes = [C.POINTER(OrcParams)] + [vp] * 23
+                getattr(L, f"orc{lat}_step_{p}").restype = C.c_int
+        L.orc27_collide_f32.argtypes = [vp] + [C.c_float] * 8 + [vp]
+        L.orc27_collide_f64.argtypes = [vp] + [C.c_double] * 8 + [vp]
+        L.orc27_upwind_B.argtypes = [C.c_int] * 10
+        L.orc27_upwind_B.restype = C.c_long
+        L.orc27_mirror.argtypes = [C.c_int, C.c_int]
         L.orc19_collide_f32.argtypes = [C.c_int, vp] + [C.c_float] * 8 + [vp]
         L.orc19_collide_f64.argtypes = [C.c_int, vp] + [C.c_double] * 8 + [vp]
         L.orc19_smagorinsky_f32.argtypes = [vp] + [C.c_float] * 8 + [C.c_int]
@@ -73,8 +79,9 @@ def initial_state(case, real=np.float32):
 
 
 def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=None, sgs=None):
-    """Run nsteps of the restated reference step.  Returns the state dict (f = current
-    populations after the swaps, plus the macro arrays)."""
+    """Run nsteps of the restated reference step (D3Q19, or D3Q27 when the case has
+    q == 27).  Returns the state dict (f = current populations after the swaps, plus the
+    macro arrays)."""
     real = np.float64 if precision == "f64" else np.float32
     a = case.arrays
     st = initial_state(case, real) if state is None else {k: v.copy() for k, v in state.items()}
@@ -82,13 +89,13 @@ def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=
     P = OrcParams(nx=case.nx, ny=case.ny, nz=case.nz, order=order, sgs=case.sgs if sgs is None else sgs,
                   collision=collision, damping=int(p.get("damping", 1)), dpdx=p.get("dpdx", 0.0),
                   LES=p.get("LES", 0.0), omegaB=p.get("omegaB", 0.1), spzWidth=p.get("spzWidth", 0.1),
-                  Cs=p.get("Cs", 0.1))
+                  Cs=p.get("Cs", 0.1), clim=p.get("clim", 0.01))
     sides = {k: np.ascontiguousarray(a[k], real) for k in ("sdf", "u0", "v0", "w0", "omega")}
     ints = {k: np.ascontiguousarray(a[k], np.int32) for k in ("boundary", "boundary1", "boundary2", "boundary3")}
     u8 = {k: np.ascontiguousarray(a[k], np.uint8) for k in ("solid", "neiSolid")}
     f = st["f"]
     ft = f.copy()  # initialization.hpp: fTmp = f
-    fn = getattr(orc(), f"orc19_step_{precision}")
+    fn = getattr(orc(), f"orc{getattr(case, 'q', 19)}_step_{precision}")
     for _ in range(nsteps):
         rc = fn(C.byref(P), _p(f), _p(ft), _p(ints["boundary"]), _p(ints["boundary1"]), _p(ints["boundary2"]),
                 _p(ints["boundary3"]), _p(sides["sdf"]), _p(u8["solid"]), _p(u8["neiSolid"]), _p(sides["u0"]),
@@ -100,6 +107,18 @@ def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=
         f, ft = ft, f
     st["f"] = f
     return st
+
+
+def collide27(ft, rho, u, v, w, tau, tauSGS=0.0, omegaB=1.0, clim=0.01, precision="f64"):
+    real = np.float64 if precision == "f64" else np.float32
+    a = np.ascontiguousarray(ft, real)
+    out = np.empty(27, real)
+    getattr(orc(), f"orc27_collide_{precision}")(_p(a), rho, u, v, w, tau, tauSGS, omegaB, clim, _p(out))
+    return out
+
+
+def run27(case, nsteps, **kw):
+    return run19(case, nsteps, **kw)
 
 
 def collide(ft, rho, u, v, w, tau, tauSGS=0.0, omegaB=0.1, dpdx=0.0, kind=COLL_RR, precision="f64"):
