@@ -92,7 +92,7 @@ enum {
  * creation of ns.cpp:385-436. */
 typedef struct {
     int abi_version; /* LBFFS_ABI_VERSION */
-    int q;           /* 19 (27 reserved) */
+    int q;           /* 19 (D3Q19_opencl) or 27 (D3Q27_opencl) */
     int nx, ny, nz;
     int storage;   /* LBFFS_STORE_* */
     int sgs;       /* LBFFS_SGS_* */
@@ -102,18 +102,20 @@ typedef struct {
     int reserved[8];
 } lbffs_desc;
 
-/* Scalars of one k_streamingCollision launch (ns.cpp:528-535, kernel.cl:3541-3545). */
+/* Scalars of one k_streamingCollision launch (D3Q19 ns.cpp:528-535, K19:3541-3545;
+ * D3Q27 ns.cpp:555-577, K27:2725-2731). */
 typedef struct {
     float dpdx;      /* body force along x, lattice units */
     float LES;       /* 1 or 0: multiplies the Smagorinsky tauSGS (WALE ignores it) */
-    float omegaB;    /* bulk relaxation rate of the RR collision */
+    float omegaB;    /* bulk relaxation rate (RR collision; omega2 of the cumulant collision) */
     float spzWidth;  /* sponge width as a fraction of the domain (tauSpongeZone) */
     float Cs;        /* Smagorinsky constant */
     int damping;     /* Smagorinsky wall damping on (1) / off (0) */
     int store_every; /* 1: write rho/u/v/w/tauSGS/Fw/G for every cell on every step
                         (reference semantics throughout); 0: only on the last step of
                         the call and for the cells later steps read */
-    int reserved[8];
+    float clim;      /* limiter of the D3Q27 cumulant collision (input.hpp default 0.01) */
+    int reserved[7];
 } lbffs_step_params;
 
 typedef struct lbffs_ctx lbffs_ctx;

@@ -47,7 +47,8 @@ class Desc(C.Structure):
 
 class StepParams(C.Structure):
     _fields_ = [("dpdx", C.c_float), ("LES", C.c_float), ("omegaB", C.c_float), ("spzWidth", C.c_float),
-                ("Cs", C.c_float), ("damping", C.c_int), ("store_every", C.c_int), ("reserved", C.c_int * 8)]
+                ("Cs", C.c_float), ("damping", C.c_int), ("store_every", C.c_int), ("clim", C.c_float),
+                ("reserved", C.c_int * 7)]
 
 
 class CaseIn(C.Structure):
@@ -114,9 +115,9 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def _params(dpdx=0.0, LES=0.0, omegaB=0.1, spzWidth=0.1, Cs=0.1, damping=1, store_every=0):
+def _params(dpdx=0.0, LES=0.0, omegaB=0.1, spzWidth=0.1, Cs=0.1, damping=1, store_every=0, clim=0.01):
     return StepParams(dpdx=dpdx, LES=LES, omegaB=omegaB, spzWidth=spzWidth, Cs=Cs, damping=damping,
-                      store_every=store_every)
+                      store_every=store_every, clim=clim)
 
 
 def probe_copy(nbytes, reps=20):

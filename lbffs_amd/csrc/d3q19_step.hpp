@@ -38,12 +38,13 @@ __host__ __device__ constexpr Real w19(int q)
     return q == 0 ? Real(1) / Real(3) : (q <= 6 ? Real(1) / Real(18) : Real(1) / Real(36));
 }
 
-template <typename Real>
-struct Step19Args {
+// Arguments of one step launch, shared by the D3Q19 and D3Q27 kernels.
+template <typename Real, int Q>
+struct StepArgs {
     uint32_t N;
     int nx, ny, nz;
     FastDiv dnx, dny;
-    int off[19];  // interior pull offset: source = ic - off[q]
+    int off[Q];  // interior pull offset: source = ic - off[q]
     const void *fsrc;
     void *fdst;
     const uint16_t *flag;
@@ -52,6 +53,7 @@ struct Step19Args {
     Real *rho_n, *u_n, *v_n, *w_n;        // macros written by this step
     Real *tau, *Fwx, *Fwy, *Fwz, *Gx, *Gy, *Gz;
     Real dpdx, LES, omegaB, spz, Cs, omega0;
+    Real clim;  // cumulant limiter (D3Q27)
     int damping;
     int mac_all;      // write rho/u/v/w for every fluid cell
     int aux_all;      // write tauSGS / Fw / G for every fluid cell
@@ -59,6 +61,9 @@ struct Step19Args {
     Real inv_spx, inv_spy, inv_spz;  // 1 / (n * spzWidth) per axis
     int tune;         // TUNE_* launch options chosen by the autotuner
 };
+
+template <typename Real>
+using Step19Args = StepArgs<Real, 19>;
 
 // launch options (uniform per launch)
 enum : int {
@@ -248,8 +253,8 @@ __device__ inline Real grad_comp(const Real *old, Real self, long ic, int i, int
 }
 
 // kernel.cl:2267-2365 waleTauSGS (Cw = 0.325)
-template <typename Real>
-__device__ inline Real wale_tau(const Step19Args<Real> &a, long ic, int i, int j, int k, Real u, Real v, Real w)
+template <typename Real, int Q>
+__device__ inline Real wale_tau(const StepArgs<Real, Q> &a, long ic, int i, int j, int k, Real u, Real v, Real w)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     Real dudx = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 0);
@@ -289,6 +294,17 @@ __device__ inline Real wale_tau(const Step19Args<Real> &a, long ic, int i, int j
 // reciprocal: one v_rcp_f32 in FP32 (<= 1 ulp), an exact quotient in FP64
 __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline double rcp(double x) { return 1.0 / x; }
+// reciprocal refined by one Newton step (e = 1 - x r, r += r e): unbiased to within
+// rounding, as a division would be.  Used for 1/rho, whose error the collision turns into
+// momentum (the output momentum is rho times the velocity it was given), so that a
+// one-sided error of the hardware reciprocal cannot accumulate over steps.
+__device__ inline float rcp_nr(float x)
+{
+    const float r = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, r, 1.0f);
+    return __builtin_fmaf(r, e, r);
+}
+__device__ inline double rcp_nr(double x) { return 1.0 / x; }
 // square root: one v_sqrt_f32 in FP32 (<= 1 ulp), the correctly rounded one in FP64
 __device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ inline double fsqrt(double x) { return sqrt(x); }
@@ -344,8 +360,8 @@ __device__ inline Real smagorinsky_tau(const Mom19<Real> &m, Real u, Real v, Rea
 }
 
 // kernel.cl:2444-2503 tauSpongeZone
-template <typename Real>
-__device__ inline Real sponge_tau(const Step19Args<Real> &a, Real tau, Real tauSGS, int i, int j, int k)
+template <typename Real, int Q>
+__device__ inline Real sponge_tau(const StepArgs<Real, Q> &a, Real tau, Real tauSGS, int i, int j, int k)
 {
     const Real spz = a.spz;
     const int m = a.sponge_mask;
@@ -666,7 +682,7 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
     // updateRhoUVW, kernel.cl:3509-3523 (u carries the half-step force shift)
     const Mom19<Real> m = moments19<Real>(ft);
     const Real rho = m.rho;
-    const Real inv_rho = rcp(rho);
+    const Real inv_rho = rcp_nr(rho);
     const Real hf = Real(0.5) * a.dpdx * inv_rho;
     const Real u = m.mx * inv_rho + hf;
     const Real v = m.my * inv_rho, w = m.mz * inv_rho;

@@ -1,0 +1,626 @@
+// d3q27_step.hpp -- the fused D3Q27 time-step kernel with the parametrized cumulant
+// collision.
+//
+// Same work decomposition and data layout as the D3Q19 kernel (one work-item per cell,
+// q-major populations, interior pulls from SGPR bases).  Per fluid cell it does what
+// k_streamingCollision of D3Q27_opencl/kernel.cl:2711-2826 does:
+//   pull streaming (faces: wrap, or reflect/mirror through wall faces),
+//   fixed-velocity / fixed-density / wall-function / outlet face conditions,
+//   interpolated bounce-back on solid links (Filippova-Hanel, Mei-Luo-Shyy for q <= 1/2)
+//   with momentum-exchange force, moments, WALE or Smagorinsky sub-grid model, sponge,
+//   and the cumulant collision of Geier et al. (2017) with the velocity shifted by half
+//   the body force.
+// The collision runs on the central moments of the 27 populations obtained by three
+// passes of 3-point transforms (one per axis, nine lines each) instead of 27x27 sums; the
+// back-transform is the reference's three passes (x, then y, then z).
+#pragma once
+
+#include "d3q19_step.hpp"
+
+namespace lbffs {
+
+// D3Q27 directions (K27:2742-2747)
+struct L27 {
+    static constexpr int cx[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1};
+    static constexpr int cy[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1};
+    static constexpr int cz[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1, 1, -1, -1, 1};
+    // opposite direction: pairs (2m-1, 2m) as in D3Q19
+    __host__ __device__ static constexpr int opp(int q) { return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1); }
+    // K27:186-528 mirrorQx/y/z: the component along the axis flipped, or the whole
+    // direction reflected when it has none
+    static constexpr int mirx[27] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 16, 15, 18, 17, 21, 22, 19, 20, 26, 25, 24, 23};
+    static constexpr int miry[27] = {0, 2, 1, 4, 3, 6, 5, 9, 10, 7, 8, 12, 11, 14, 13, 18, 17, 16, 15, 23, 24, 26, 25, 19, 20, 22, 21};
+    static constexpr int mirz[27] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 13, 14, 11, 12, 17, 18, 15, 16, 25, 26, 24, 23, 22, 21, 19, 20};
+    // slot s = 9 (cx+1) + 3 (cy+1) + (cz+1) of each direction, and the direction of each slot
+    static constexpr int slot[27] = {13, 22, 4, 16, 10, 14, 12, 25, 1, 19, 7, 23, 3, 21, 5, 17, 9, 15, 11, 26, 0, 8, 18, 20, 6, 24, 2};
+    static constexpr int q_of_slot[27] = {20, 8, 26, 12, 2, 14, 24, 10, 21, 16, 4, 18, 6, 0, 5, 17, 3, 15, 22, 9, 23, 13, 1, 11, 25, 7, 19};
+};
+
+namespace detail {
+constexpr bool l27_tables_consistent()
+{
+    for (int q = 0; q < 27; q++) {
+        const int x = L27::cx[q], y = L27::cy[q], z = L27::cz[q];
+        const int o = L27::opp(q);
+        if (L27::cx[o] != -x || L27::cy[o] != -y || L27::cz[o] != -z) return false;
+        const int mx = L27::mirx[q], my = L27::miry[q], mz = L27::mirz[q];
+        if (x != 0 && (L27::cx[mx] != -x || L27::cy[mx] != y || L27::cz[mx] != z)) return false;
+        if (y != 0 && (L27::cx[my] != x || L27::cy[my] != -y || L27::cz[my] != z)) return false;
+        if (z != 0 && (L27::cx[mz] != x || L27::cy[mz] != y || L27::cz[mz] != -z)) return false;
+        if (x == 0 && mx != o) return false;
+        if (y == 0 && my != o) return false;
+        if (z == 0 && mz != o) return false;
+        const int s = 9 * (x + 1) + 3 * (y + 1) + (z + 1);
+        if (L27::slot[q] != s || L27::q_of_slot[s] != q) return false;
+    }
+    return true;
+}
+static_assert(l27_tables_consistent(), "D3Q27 direction tables");
+}  // namespace detail
+
+// K27:2740 weights 8/27, 2/27, 1/54, 1/216
+template <typename Real>
+__host__ __device__ constexpr Real w27(int q)
+{
+    return q == 0 ? Real(8) / Real(27)
+                  : (q <= 6 ? Real(2) / Real(27) : (q <= 18 ? Real(1) / Real(54) : Real(1) / Real(216)));
+}
+
+// lattice weight of direction q for either lattice
+template <typename Real, int Q>
+__host__ __device__ constexpr Real wq(int q)
+{
+    return Q == 27 ? w27<Real>(q) : w19<Real>(q);
+}
+
+template <int DIR>
+__host__ __device__ constexpr int c27(int q)
+{
+    return DIR == 0 ? L27::cx[q] : (DIR == 1 ? L27::cy[q] : L27::cz[q]);
+}
+
+// K27:530-549 upwindID_B: pull source of a face cell, -1 through a wall face
+__device__ inline long face_upwind27(int q, int i, int j, int k, int nx, int ny, int nz, int b1, int b2, int b3)
+{
+    int si = i - L27::cx[q], sj = j - L27::cy[q], sk = k - L27::cz[q];
+    if (si < 0 || si >= nx) {
+        if (is_wall_code(b1)) return -1;
+        si = (si < 0) ? nx - 1 : 0;
+    }
+    if (sj < 0 || sj >= ny) {
+        if (is_wall_code(b2)) return -1;
+        sj = (sj < 0) ? ny - 1 : 0;
+    }
+    if (sk < 0 || sk >= nz) {
+        if (is_wall_code(b3)) return -1;
+        sk = (sk < 0) ? nz - 1 : 0;
+    }
+    return (long)si + (long)nx * ((long)sj + (long)ny * sk);
+}
+
+// Zou-He wall density numerator of a face: tangential populations plus twice those
+// leaving through it
+template <int DIR, typename Real>
+__device__ inline Real zouhe27(const Real *ft, bool lo)
+{
+    Real t = Real(0), om = Real(0), op = Real(0);
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        const int c = c27<DIR>(q);
+        if (c == 0) t += ft[q];
+        else if (c < 0) om += ft[q];
+        else op += ft[q];
+    }
+    return t + Real(2) * (lo ? om : op);
+}
+
+// Ladd moving-wall correction ft_q += 6 w_q rho_w (c_q . u_w) of the nine populations
+// entering through a face.  The nzMax face of the reference divides its four
+// corner-diagonal terms by 9 instead of 36 (K27:1376-1379, 1815-1818): zmax_quirk.
+template <int DIR, typename Real>
+__device__ inline void ladd27(Real *ft, bool lo, Real rhow, Real uw, Real vw, Real ww, bool zmax_quirk)
+{
+#pragma unroll
+    for (int q = 1; q < 27; q++) {
+        const int c = c27<DIR>(q);
+        if (c == 0) continue;
+        const Real cu = Real(L27::cx[q]) * uw + Real(L27::cy[q]) * vw + Real(L27::cz[q]) * ww;
+        Real den = q <= 6 ? Real(9) / Real(4) : (q <= 18 ? Real(9) : Real(36));
+        if (q > 18) den = zmax_quirk ? Real(9) : den;
+        const Real d = rhow * cu / den;
+        const bool enters = (c > 0) ? lo : !lo;
+        ft[q] = enters ? ft[q] + d : ft[q];
+    }
+}
+
+template <int DIR, typename S, typename Real>
+__device__ inline void fixed_velocity27(const StepArgs<Real, 27> &a, Real *ft, long ic, int pos, long st, Real u0,
+                                        Real v0, Real w0)
+{
+    const bool lo = (pos == 0);
+    const Real un = DIR == 0 ? u0 : (DIR == 1 ? v0 : w0);
+    Real rhow = zouhe27<DIR>(ft, lo) / (lo ? Real(1) - un : Real(1) + un);
+    if (lo && un < Real(0)) rhow = a.rho_o[ic + st];
+    // the x face takes the cell's own stored density (K27:1272-1276), y and z the inner cell
+    if (!lo && un > Real(0)) rhow = (DIR == 0) ? a.rho_o[ic] : a.rho_o[ic - st];
+    ladd27<DIR>(ft, lo, rhow, u0, v0, w0, DIR == 2 && !lo);
+}
+
+template <int DIR, typename Real>
+__device__ inline void fixed_density27(const StepArgs<Real, 27> &a, Real *ft, long ic, int pos, int n, long st)
+{
+    const Real uc = a.u_o[ic], vc = a.v_o[ic], wc = a.w_o[ic];
+    const long inner = (pos == 0) ? ic + st : ic + (long)(n - 2 - pos) * st;
+    const Real ue = uc + Real(0.5) * (uc - a.u_o[inner]);
+    const Real ve = vc + Real(0.5) * (vc - a.v_o[inner]);
+    const Real we = wc + Real(0.5) * (wc - a.w_o[inner]);
+    const Real uSqr = ue * ue + ve * ve + we * we;
+    const bool atlo = (pos == 0), athi = (pos == n - 1);
+#pragma unroll
+    for (int q = 1; q < 27; q++) {
+        const int c = c27<DIR>(q);
+        if (c == 0) continue;
+        const bool hit = (c > 0) ? atlo : athi;
+        const Real uDotC = ue * Real(L27::cx[q]) + ve * Real(L27::cy[q]) + we * Real(L27::cz[q]);
+        const Real nv = -ft[q] + Real(2) * w27<Real>(q) * Real(1) * (Real(1) + Real(4.5) * uDotC * uDotC - Real(1.5) * uSqr);
+        ft[q] = hit ? nv : ft[q];
+    }
+}
+
+template <int DIR, typename Real>
+__device__ inline void wall_function27(const StepArgs<Real, 27> &a, Real *ft, long ic, int pos, int nz, long st,
+                                       Real nu, Real Fx, Real Fy, Real Fz)
+{
+    const bool lo = (pos == 0);
+    const Real rhow = zouhe27<DIR>(ft, lo);
+    // neighbour sampled for the wall law: the next cell inward; the z faces of a 2-D
+    // lattice sample the cell itself
+    long nb = lo ? ic + st : ic - st;
+    if (DIR == 2) nb = (nz - 1 != 0) ? ic + st : ic;
+    const Real *t1 = DIR == 0 ? a.v_o : a.u_o;
+    const Real *t2 = DIR == 2 ? a.v_o : a.w_o;
+    Real p = t1[nb], s = t2[nb];
+    Real magUp = sqrt(p * p + s * s);
+    Real ut1 = 0, ut2 = 0;
+    if (magUp != Real(0)) {
+        const Real tauw = calc_tauw<Real>(rhow, magUp, nu);
+        p = t1[ic];
+        s = t2[ic];
+        magUp = sqrt(p * p + s * s);
+        if ((DIR == 2 && lo) || magUp != Real(0)) {  // the k == 0 branch has no zero guard
+            const Real e1 = p / magUp, e2 = s / magUp;
+            const Real Fw1 = DIR == 0 ? Fy : Fx;
+            const Real Fw2 = DIR == 2 ? Fy : Fz;
+            ut1 = Real(-3) * (tauw * e1 - Fw1) / rhow;
+            ut2 = Real(-3) * (tauw * e2 - Fw2) / rhow;
+        }
+    }
+    const Real uw = DIR == 0 ? Real(0) : ut1;
+    const Real vw = DIR == 0 ? ut1 : (DIR == 1 ? Real(0) : ut2);
+    const Real ww = DIR == 2 ? Real(0) : ut2;
+    ladd27<DIR>(ft, lo, rhow, uw, vw, ww, DIR == 2 && !lo);
+}
+
+// Boundary handling of a domain-face cell (K27:2764-2777): streaming with walls, then
+// the face conditions in the reference's order.
+template <typename S, typename Real>
+__device__ __forceinline__ void face_cell27(const StepArgs<Real, 27> &a, Real *ft, uint32_t fl, long ic, int i, int j,
+                                            int k, Real omega)
+{
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const long N = a.N;
+    const S *fs = (const S *)a.fsrc;
+    const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
+    Real Fx = 0, Fy = 0, Fz = 0;  // wall momentum of this step's domain-face links
+    ft[0] = Codec<S, Real, 27>::load(fs + ic, 0, w27<Real>(0));
+#pragma unroll
+    for (int q = 1; q < 27; q++) {
+        const long src = face_upwind27(q, i, j, k, nx, ny, nz, b1, b2, b3);
+        if (src >= 0) {
+            ft[q] = Codec<S, Real, 27>::load(fs + (long)q * N + src, q, w27<Real>(q));
+        } else {
+            // reflected population; a Symmetry face code mirrors instead, the z test
+            // overriding the y test overriding the x test (K27:1133-1145)
+            int qb = L27::opp(q);
+            qb = (b1 == BC_SYMMETRY) ? L27::mirx[q] : qb;
+            qb = (b2 == BC_SYMMETRY) ? L27::miry[q] : qb;
+            qb = (b3 == BC_SYMMETRY) ? L27::mirz[q] : qb;
+            ft[q] = Codec<S, Real, 27>::load(fs + (long)qb * N + ic, qb, w27<Real>(qb));
+            Fx += Real(-2) * ft[q] * Real(L27::cx[q]);
+            Fy += Real(-2) * ft[q] * Real(L27::cy[q]);
+            Fz += Real(-2) * ft[q] * Real(L27::cz[q]);
+        }
+    }
+    const int corner = corner_flag(b3, i, j, k, nx, ny, nz);
+    const long sy = nx, sz = (long)nx * ny;
+
+    // fixedVelocityBC, K27:1243-1383
+    if (corner == 0 && (b1 == BC_FIXEDVELOCITY || b2 == BC_FIXEDVELOCITY || b3 == BC_FIXEDVELOCITY)) {
+        const Real u0 = a.u0[ic], v0 = a.v0[ic], w0 = a.w0[ic];
+        if (b1 == BC_FIXEDVELOCITY && (i == 0 || i == nx - 1)) fixed_velocity27<0, S>(a, ft, ic, i, 1, u0, v0, w0);
+        if (b2 == BC_FIXEDVELOCITY && (j == 0 || j == ny - 1)) fixed_velocity27<1, S>(a, ft, ic, j, sy, u0, v0, w0);
+        if (b3 == BC_FIXEDVELOCITY && (k == 0 || k == nz - 1)) fixed_velocity27<2, S>(a, ft, ic, k, sz, u0, v0, w0);
+    }
+    // fixedDensityBC with rho_w = 1, K27:1385-1485
+    if (corner == 0 && (b1 == BC_FIXEDDENSITY || b2 == BC_FIXEDDENSITY || b3 == BC_FIXEDDENSITY)) {
+        if (b1 == BC_FIXEDDENSITY) fixed_density27<0>(a, ft, ic, i, nx, 1);
+        if (b2 == BC_FIXEDDENSITY) fixed_density27<1>(a, ft, ic, j, ny, sy);
+        if (b3 == BC_FIXEDDENSITY) fixed_density27<2>(a, ft, ic, k, nz, sz);
+    }
+    // wallFunctionBC, K27:1524-1821 (stationary wall, tangential slip velocity)
+    if (corner == 0 && (b1 == BC_WALLFUNCTION || b2 == BC_WALLFUNCTION || b3 == BC_WALLFUNCTION)) {
+        const Real nu = (Real(1) / omega - Real(0.5)) / Real(3);
+        if (b1 == BC_WALLFUNCTION && (i == 0 || i == nx - 1)) wall_function27<0>(a, ft, ic, i, nz, 1, nu, Fx, Fy, Fz);
+        if (b2 == BC_WALLFUNCTION && (j == 0 || j == ny - 1)) wall_function27<1>(a, ft, ic, j, nz, sy, nu, Fx, Fy, Fz);
+        if (b3 == BC_WALLFUNCTION && (k == 0 || k == nz - 1)) wall_function27<2>(a, ft, ic, k, nz, sz, nu, Fx, Fy, Fz);
+    }
+    // outflowBC, K27:1900-1991: convective outlet with the cell's last velocity
+    if (b1 == BC_OUTLET || b2 == BC_OUTLET || b3 == BC_OUTLET) {
+        const Real is3 = Real(0.57735026918962576);
+        const Real uo = a.u_o[ic], vo = a.v_o[ic], wo = a.w_o[ic];
+#pragma unroll
+        for (int q = 1; q < 27; q++) {
+            const int cx = L27::cx[q], cy = L27::cy[q], cz = L27::cz[q];
+            const S *fq = fs + (long)q * N;
+#define OUTF(cond, inner, un)                                                                          \
+    if (cond) {                                                                                        \
+        const Real fi = Codec<S, Real, 27>::load(fq + (inner), q, w27<Real>(q));                      \
+        const Real fc = Codec<S, Real, 27>::load(fq + ic, q, w27<Real>(q));                           \
+        ft[q] = (is3 - (un)) * fi + (Real(1) - (is3 - (un))) * fc;                                      \
+    }
+            OUTF(i == 0 && b1 == BC_OUTLET && cx == 1, ic + 1, uo)
+            OUTF(i == nx - 1 && b1 == BC_OUTLET && cx == -1, ic - 1, uo)
+            OUTF(j == 0 && b2 == BC_OUTLET && cy == 1, ic + sy, vo)
+            OUTF(j == ny - 1 && b2 == BC_OUTLET && cy == -1, ic - sy, vo)
+            OUTF(k == 0 && b3 == BC_OUTLET && cz == 1, ic + sz, wo)
+            OUTF(k == nz - 1 && b3 == BC_OUTLET && cz == -1, ic - sz, wo)
+#undef OUTF
+        }
+    }
+}
+
+// Density, momentum and raw second moments of the 27 streamed populations.
+template <typename Real>
+__device__ inline Mom19<Real> moments27(const Real *ft)
+{
+    Mom19<Real> m;
+    Real r = Real(0), mx = Real(0), my = Real(0), mz = Real(0);
+    Real sxx = Real(0), syy = Real(0), szz = Real(0), sxy = Real(0), sxz = Real(0), syz = Real(0);
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        const int x = L27::cx[q], y = L27::cy[q], z = L27::cz[q];
+        r += ft[q];
+        if (x) mx += x > 0 ? ft[q] : -ft[q];
+        if (y) my += y > 0 ? ft[q] : -ft[q];
+        if (z) mz += z > 0 ? ft[q] : -ft[q];
+        if (x) sxx += ft[q];
+        if (y) syy += ft[q];
+        if (z) szz += ft[q];
+        if (x && y) sxy += (x * y > 0) ? ft[q] : -ft[q];
+        if (x && z) sxz += (x * z > 0) ? ft[q] : -ft[q];
+        if (y && z) syz += (y * z > 0) ? ft[q] : -ft[q];
+    }
+    m.rho = r;
+    m.mx = mx;
+    m.my = my;
+    m.mz = mz;
+    m.sxx = sxx;
+    m.syy = syy;
+    m.szz = szz;
+    m.sxy = sxy;
+    m.sxz = sxz;
+    m.syz = syz;
+    return m;
+}
+
+// 3-point central-moment transform along one axis: (f(-1), f(0), f(+1)) ->
+// (k0, k1, k2) = sum_c (c - u)^n f(c), n = 0, 1, 2
+template <typename Real>
+__device__ inline void fwd3(Real &a, Real &b, Real &c, Real u)
+{
+    const Real s = c + a, d = c - a;
+    const Real k0 = s + b;
+    const Real k1 = d - u * k0;
+    const Real k2 = s - u * (d + k1);
+    a = k0;
+    b = k1;
+    c = k2;
+}
+
+// inverse of fwd3 (K27:2563-2662): (k0, k1, k2) -> (f(-1), f(0), f(+1))
+template <typename Real>
+__device__ inline void bwd3(Real &a, Real &b, Real &c, Real u)
+{
+    const Real k0 = a, k1 = b, k2 = c;
+    const Real uu = u * u;
+    b = k0 * (Real(1) - uu) - Real(2) * u * k1 - k2;
+    a = Real(0.5) * (k0 * (uu - u) + k1 * (Real(2) * u - Real(1)) + k2);
+    c = Real(0.5) * (k0 * (uu + u) + k1 * (Real(2) * u + Real(1)) + k2);
+}
+
+// K27:2352-2691 collisionCumulantGeier: cumulants of the central moments about the
+// force-shifted velocity, relaxation with omega1 = omegaEff, omega2 = omegaB, omega3..5
+// from the parametrization of Geier et al. (2017) with the limiter clim, the A/B
+// fourth-order corrections, and back.  The back-transform products of Ks122, Ks212 and
+// Ks221 use Ks022, Ks202 and Ks220 where the forward ones use K002, K200 and K020, as the
+// reference does (K27:2557-2559).  ft (by direction) is replaced by the post-collision
+// populations.
+template <typename Real>
+__device__ inline void cumulant27(Real *ft, Real rho, Real u, Real v, Real w, Real omegaEff, Real omegaB, Real clim)
+{
+    Real K[27];
+#pragma unroll
+    for (int s = 0; s < 27; s++) K[s] = ft[L27::q_of_slot[s]];
+    // central moments: z lines, y lines, x lines; afterwards K[9a+3b+c] = K_abc
+#pragma unroll
+    for (int p = 0; p < 9; p++) fwd3(K[3 * p], K[3 * p + 1], K[3 * p + 2], w);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) fwd3(K[9 * a + c], K[9 * a + 3 + c], K[9 * a + 6 + c], v);
+#pragma unroll
+    for (int p = 0; p < 9; p++) fwd3(K[p], K[9 + p], K[18 + p], u);
+#define KM(a, b, c) K[9 * (a) + 3 * (b) + (c)]
+    const Real K000 = rho;
+    const Real K100 = KM(1, 0, 0), K010 = KM(0, 1, 0), K001 = KM(0, 0, 1);
+    const Real K200 = KM(2, 0, 0), K020 = KM(0, 2, 0), K002 = KM(0, 0, 2);
+    const Real K110 = KM(1, 1, 0), K101 = KM(1, 0, 1), K011 = KM(0, 1, 1);
+    const Real K210 = KM(2, 1, 0), K201 = KM(2, 0, 1), K021 = KM(0, 2, 1);
+    const Real K120 = KM(1, 2, 0), K102 = KM(1, 0, 2), K012 = KM(0, 1, 2), K111 = KM(1, 1, 1);
+    const Real K220 = KM(2, 2, 0), K202 = KM(2, 0, 2), K022 = KM(0, 2, 2);
+    const Real K211 = KM(2, 1, 1), K121 = KM(1, 2, 1), K112 = KM(1, 1, 2);
+    const Real K221 = KM(2, 2, 1), K212 = KM(2, 1, 2), K122 = KM(1, 2, 2), K222 = KM(2, 2, 2);
+
+    const Real invRho = rcp_nr(rho);
+    // cumulants (orders 2 and 3 equal the central moments)
+    const Real C200 = K200, C020 = K020, C002 = K002, C110 = K110, C101 = K101, C011 = K011;
+    const Real C210 = K210, C201 = K201, C021 = K021, C120 = K120, C102 = K102, C012 = K012, C111 = K111;
+    const Real C211 = K211 - (K200 * K011 + Real(2) * K110 * K101) * invRho;
+    const Real C121 = K121 - (K020 * K101 + Real(2) * K110 * K011) * invRho;
+    const Real C112 = K112 - (K002 * K110 + Real(2) * K101 * K011) * invRho;
+    const Real C220 = K220 - (K200 * K020 + Real(2) * K110 * K110) * invRho;
+    const Real C202 = K202 - (K002 * K200 + Real(2) * K101 * K101) * invRho;
+    const Real C022 = K022 - (K020 * K002 + Real(2) * K011 * K011) * invRho;
+    const Real C122 = K122 - (K002 * K120 + K020 * K102 + Real(4) * K011 * K111 + Real(2) * (K101 * K021 + K110 * K012)) * invRho;
+    const Real C212 = K212 - (K200 * K012 + K002 * K210 + Real(4) * K101 * K111 + Real(2) * (K110 * K102 + K011 * K201)) * invRho;
+    const Real C221 = K221 - (K020 * K201 + K200 * K021 + Real(4) * K110 * K111 + Real(2) * (K011 * K210 + K101 * K120)) * invRho;
+    const Real C222 = K222
+                      - (Real(4) * K111 * K111 + K200 * K022 + K020 * K202 + K002 * K220
+                         + Real(4) * (K011 * K211 + K101 * K121 + K110 * K112)
+                         + Real(2) * (K120 * K102 + K210 * K012 + K201 * K021)) * invRho
+                      + (Real(16) * K110 * K101 * K011 + Real(4) * (K101 * K101 * K020 + K011 * K011 * K200 + K110 * K110 * K002)
+                         + Real(2) * K200 * K020 * K002) * invRho * invRho;
+
+    // relaxation rates (Geier et al. 2017, eqs. for omega3..omega5, A, B)
+    const Real o1 = omegaEff, o2 = omegaB;
+    const Real o3 = Real(8) * (o1 - Real(2)) * (o2 * (Real(3) * o1 - Real(1)) - Real(5) * o1)
+                    * rcp(Real(8) * (Real(5) - Real(2) * o1) * o1 + o2 * (Real(8) + o1 * (Real(9) * o1 - Real(26))));
+    const Real o4 = Real(8) * (o1 - Real(2)) * (o1 + o2 * (Real(3) * o1 - Real(7)))
+                    * rcp(o2 * (Real(56) - Real(42) * o1 + Real(9) * o1 * o1) - Real(8) * o1);
+    const Real o5 = Real(24) * (o1 - Real(2))
+                    * (Real(4) * o1 * o1 + o1 * o2 * (Real(18) - Real(13) * o1) + o2 * o2 * (Real(2) + o1 * (Real(6) * o1 - Real(11))))
+                    * rcp(Real(16) * o1 * o1 * (o1 - Real(6)) - Real(2) * o1 * o2 * (Real(216) + Real(5) * o1 * (Real(9) * o1 - Real(46)))
+                          + o2 * o2 * (o1 * (Real(3) * o1 - Real(10)) * (Real(15) * o1 - Real(28)) - Real(48)));
+    const Real climRho = clim * rho;
+    auto lim = [&](Real om, Real c) -> Real {
+        const Real ac = fabs(c);
+        return ac > Real(0) ? om + (Real(1) - om) * ac * rcp(ac + climRho) : Real(1);
+    };
+    const Real o3a = lim(o3, C120 + C102), o3b = lim(o3, C210 + C012), o3c = lim(o3, C201 + C021);
+    const Real o4a = lim(o4, C120 - C102), o4b = lim(o4, C210 - C012), o4c = lim(o4, C201 - C021);
+    const Real o5l = lim(o5, C111);
+    const Real denAB = rcp((o1 - o2) * (o2 * (Real(2) + Real(3) * o1) - Real(8) * o1));
+    Real A = (Real(4) * o1 * o1 + Real(2) * o1 * o2 * (o1 - Real(6)) + o2 * o2 * (o1 * (Real(10) - Real(3) * o1) - Real(4))) * denAB;
+    Real B = (Real(4) * o1 * o2 * (Real(9) * o1 - Real(16)) - Real(4) * o1 * o1 - Real(2) * o2 * o2 * (Real(2) + Real(9) * o1 * (o1 - Real(2))))
+             * denAB * (Real(1) / Real(3));
+    A = fmax(A, Real(0));
+    B = fmax(B, Real(0));
+
+    // second order: shear rates relax with omega1, the trace with omega2
+    const Real Cs110 = (Real(1) - o1) * C110, Cs101 = (Real(1) - o1) * C101, Cs011 = (Real(1) - o1) * C011;
+    const Real Dxu = Real(-0.5) * (o1 * (Real(2) * C200 - C020 - C002) + o2 * (C200 + C020 + C002 - K000)) * invRho;
+    const Real Dyv = Dxu + Real(1.5) * o1 * (C200 - C020) * invRho;
+    const Real Dzw = Dxu + Real(1.5) * o1 * (C200 - C002) * invRho;
+    const Real Dxv_Dyu = Real(-3) * o1 * C110 * invRho, Dxw_Dzu = Real(-3) * o1 * C101 * invRho;
+    const Real Dyw_Dzv = Real(-3) * o1 * C011 * invRho;
+    Real X1 = (Real(1) - o1) * (C200 - C020) - Real(3) * rho * (Real(1) - Real(0.5) * o1) * (u * u * Dxu - v * v * Dyv);
+    Real X2 = (Real(1) - o1) * (C200 - C002) - Real(3) * rho * (Real(1) - Real(0.5) * o1) * (u * u * Dxu - w * w * Dzw);
+    Real X3 = K000 * o2 + (Real(1) - o2) * (C200 + C020 + C002)
+              - Real(3) * rho * (Real(1) - Real(0.5) * o2) * (u * u * Dxu + v * v * Dyv + w * w * Dzw);
+    const Real Cs200 = (X1 + X2 + X3) / Real(3), Cs020 = Cs200 - X1, Cs002 = Cs200 - X2;
+    // third order
+    X1 = (Real(1) - o3a) * (C120 + C102);
+    X2 = (Real(1) - o4a) * (C120 - C102);
+    const Real Cs120 = Real(0.5) * (X1 + X2), Cs102 = Real(0.5) * (X1 - X2);
+    X1 = (Real(1) - o3b) * (C210 + C012);
+    X2 = (Real(1) - o4b) * (C210 - C012);
+    const Real Cs210 = Real(0.5) * (X1 + X2), Cs012 = Real(0.5) * (X1 - X2);
+    X1 = (Real(1) - o3c) * (C201 + C021);
+    X2 = (Real(1) - o4c) * (C201 - C021);
+    const Real Cs201 = Real(0.5) * (X1 + X2), Cs021 = Real(0.5) * (X1 - X2);
+    const Real Cs111 = (Real(1) - o5l) * C111;
+    // fourth order with omega6..8 = 1 (only the A/B terms remain); fifth, sixth order -> 0
+    const Real vis = rcp(o1) - Real(0.5);
+    X1 = Real(2) * vis * A * rho * (Dxu - Real(2) * Dyv + Dzw) / Real(3);
+    X2 = Real(2) * vis * A * rho * (Dxu + Dyv - Real(2) * Dzw) / Real(3);
+    X3 = Real(-4) * vis * A * rho * (Dxu + Dyv + Dzw) / Real(3);
+    const Real Cs220 = (X1 + X2 + X3) / Real(3), Cs202 = -(X1 - X3) / Real(3), Cs022 = -(X2 - X3) / Real(3);
+    const Real Cs211 = -vis * B * rho * Dyw_Dzv / Real(3);
+    const Real Cs121 = -vis * B * rho * Dxw_Dzu / Real(3);
+    const Real Cs112 = -vis * B * rho * Dxv_Dyu / Real(3);
+
+    // back to central moments
+    const Real Ks200 = Cs200, Ks020 = Cs020, Ks002 = Cs002, Ks110 = Cs110, Ks101 = Cs101, Ks011 = Cs011;
+    const Real Ks111 = Cs111, Ks210 = Cs210, Ks201 = Cs201, Ks021 = Cs021, Ks120 = Cs120, Ks102 = Cs102;
+    const Real Ks012 = Cs012;
+    const Real Ks211 = Cs211 + (Ks200 * Ks011 + Real(2) * Ks110 * Ks101) * invRho;
+    const Real Ks121 = Cs121 + (Ks020 * Ks101 + Real(2) * Ks011 * Ks110) * invRho;
+    const Real Ks112 = Cs112 + (Ks002 * Ks110 + Real(2) * Ks101 * Ks011) * invRho;
+    const Real Ks220 = Cs220 + (Ks200 * Ks020 + Real(2) * Ks110 * Ks110) * invRho;
+    const Real Ks202 = Cs202 + (Ks002 * Ks200 + Real(2) * Ks101 * Ks101) * invRho;
+    const Real Ks022 = Cs022 + (Ks020 * Ks002 + Real(2) * Ks011 * Ks011) * invRho;
+    const Real Ks122 = (Ks022 * Ks120 + Ks020 * Ks102 + Real(4) * Ks011 * Ks111 + Real(2) * (Ks101 * Ks021 + Ks110 * Ks012)) * invRho;
+    const Real Ks212 = (Ks202 * Ks012 + Ks002 * Ks210 + Real(4) * Ks101 * Ks111 + Real(2) * (Ks110 * Ks102 + Ks011 * Ks201)) * invRho;
+    const Real Ks221 = (Ks220 * Ks201 + Ks200 * Ks021 + Real(4) * Ks110 * Ks111 + Real(2) * (Ks011 * Ks210 + Ks101 * Ks120)) * invRho;
+    const Real Ks222 = (Real(4) * Ks111 * Ks111 + Ks200 * Ks022 + Ks020 * Ks202 + Ks002 * Ks220
+                        + Real(4) * (Ks011 * Ks211 + Ks101 * Ks121 + Ks110 * Ks112)
+                        + Real(2) * (Ks120 * Ks102 + Ks210 * Ks012 + Ks201 * Ks021)) * invRho
+                       - (Real(16) * Ks110 * Ks101 * Ks011
+                          + Real(4) * (Ks101 * Ks101 * Ks020 + Ks011 * Ks011 * Ks200 + Ks110 * Ks110 * Ks002)
+                          + Real(2) * Ks200 * Ks020 * Ks002) * invRho * invRho;
+    KM(0, 0, 0) = K000;
+    KM(1, 0, 0) = -K100;
+    KM(0, 1, 0) = -K010;
+    KM(0, 0, 1) = -K001;
+    KM(2, 0, 0) = Ks200; KM(0, 2, 0) = Ks020; KM(0, 0, 2) = Ks002;
+    KM(1, 1, 0) = Ks110; KM(1, 0, 1) = Ks101; KM(0, 1, 1) = Ks011;
+    KM(2, 1, 0) = Ks210; KM(2, 0, 1) = Ks201; KM(0, 2, 1) = Ks021;
+    KM(1, 2, 0) = Ks120; KM(1, 0, 2) = Ks102; KM(0, 1, 2) = Ks012; KM(1, 1, 1) = Ks111;
+    KM(2, 2, 0) = Ks220; KM(2, 0, 2) = Ks202; KM(0, 2, 2) = Ks022;
+    KM(2, 1, 1) = Ks211; KM(1, 2, 1) = Ks121; KM(1, 1, 2) = Ks112;
+    KM(2, 2, 1) = Ks221; KM(2, 1, 2) = Ks212; KM(1, 2, 2) = Ks122; KM(2, 2, 2) = Ks222;
+#undef KM
+    // populations: x lines, y lines, z lines (K27:2563-2662)
+#pragma unroll
+    for (int p = 0; p < 9; p++) bwd3(K[p], K[9 + p], K[18 + p], u);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) bwd3(K[9 * a + c], K[9 * a + 3 + c], K[9 * a + 6 + c], v);
+#pragma unroll
+    for (int p = 0; p < 9; p++) bwd3(K[3 * p], K[3 * p + 1], K[3 * p + 2], w);
+#pragma unroll
+    for (int q = 0; q < 27; q++) ft[q] = K[L27::slot[q]];
+}
+
+template <typename S, typename Real, int SGS>
+__global__ void __launch_bounds__(256) step27_kernel(const StepArgs<Real, 27> a)
+{
+    const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t icu = blk * blockDim.x + threadIdx.x;
+    if (icu >= a.N) return;
+    const uint32_t fl = a.flag[icu];
+    if (fl & F_SOLID) return;
+    const long ic = icu;
+    const uint32_t N = a.N;
+    const uint32_t t = fdiv(icu, a.dnx);
+    const int i = (int)(icu - t * (uint32_t)a.nx);
+    const uint32_t kk = fdiv(t, a.dny);
+    const int j = (int)(t - kk * (uint32_t)a.ny);
+    const int k = (int)kk;
+    const S *fs = (const S *)a.fsrc;
+    S *fd = (S *)a.fdst;
+
+    const Real omega = (fl & F_OMEGAVAR) ? a.omega[ic] : a.omega0;
+    Real ft[27];
+    if (!(fl & F_FACE)) {
+        // interior pull: per-direction SGPR base + 32-bit byte offset; a 2-D lattice
+        // keeps the z links in the layer (K27:797-820)
+        const uint32_t boff = icu * (uint32_t)sizeof(S);
+        const uint32_t sy = (uint32_t)a.nx * (uint32_t)sizeof(S);
+        const uint32_t sz = (a.nz == 1) ? 0u : (uint32_t)a.nx * (uint32_t)a.ny * (uint32_t)sizeof(S);
+#pragma unroll
+        for (int q = 0; q < 27; q++) {
+            const char *fq = (const char *)(fs + (size_t)q * N);
+            const uint32_t yz = boff - (uint32_t)L27::cy[q] * sy - (uint32_t)L27::cz[q] * sz;
+            const uint32_t src = yz - (uint32_t)(L27::cx[q] * (int)sizeof(S));
+            ft[q] = Codec<S, Real, 27>::load((const S *)(fq + src), q, w27<Real>(q));
+        }
+    } else {
+        face_cell27<S, Real>(a, ft, fl, ic, i, j, k, omega);
+    }
+
+    // internalWallBC, K27:1823-1897: interpolated bounce-back on solid links with the
+    // cell's moments and tauSGS of the previous step; q from the signed distances.
+    // A NEISOLID face cell lies only on faces that wrap (the flags refuse one on a wall
+    // face, where the reference reads solid[-1]), so every link below has a source.
+    Real Fx = 0, Fy = 0, Fz = 0;
+    if (fl & F_NEISOLID) {
+        const bool face = (fl & F_FACE) != 0;
+        const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
+        const Real rhoo = a.rho_o[ic], uo = a.u_o[ic], vo = a.v_o[ic], wo = a.w_o[ic];
+        const Real omegaW = Real(1) / (Real(1) / omega + a.tau[ic]);
+        const Real uSqr = uo * uo + vo * vo + wo * wo;
+        const Real sdf0 = fabs(a.sdf[ic]);
+#pragma unroll
+        for (int q = 1; q < 27; q++) {
+            const int qb = L27::opp(q);
+            const long src = face ? face_upwind27(q, i, j, k, a.nx, a.ny, a.nz, b1, b2, b3) : ic - a.off[q];
+            if (src >= 0 && (a.flag[src] & F_SOLID)) {
+                const Real fb = Codec<S, Real, 27>::load(fs + (size_t)qb * N + icu, qb, w27<Real>(qb));
+                const Real cx = Real(L27::cx[q]), cy = Real(L27::cy[q]), cz = Real(L27::cz[q]);
+                const Real qf = sdf0 / (sdf0 + fabs(a.sdf[src]));
+                const Real uDotC = -uo * cx - vo * cy - wo * cz;
+                Real feq, chi;
+                if (qf <= Real(0.5)) {
+                    // the fluid cell beyond, upID[qbb] = x + c_q
+                    long nb = face ? face_upwind27(qb, i, j, k, a.nx, a.ny, a.nz, b1, b2, b3) : ic + a.off[q];
+                    nb = nb < 0 ? ic : nb;
+                    const Real ufDotC = -a.u_o[nb] * cx - a.v_o[nb] * cy - a.w_o[nb] * cz;
+                    feq = (rhoo + Real(3) * ufDotC + Real(4.5) * uDotC * uDotC - Real(1.5) * uSqr) * w27<Real>(q);
+                    chi = omegaW * (Real(2) * qf - Real(1)) / (Real(1) - Real(2) * omegaW);
+                } else {
+                    feq = (rhoo + Real(3) * (Real(1) - Real(1) / qf) * uDotC + Real(4.5) * uDotC * uDotC
+                           - Real(1.5) * uSqr) * w27<Real>(q);
+                    chi = omegaW * (Real(2) * qf - Real(1));
+                }
+                ft[q] = (Real(1) - chi) * fb + chi * feq;
+                Fx += -(fb + ft[q]) * cx;
+                Fy += -(fb + ft[q]) * cy;
+                Fz += -(fb + ft[q]) * cz;
+            }
+        }
+    }
+
+    // updateRhoUVW (u carries the half-step force shift)
+    const Mom19<Real> m = moments27<Real>(ft);
+    const Real rho = m.rho;
+    const Real inv_rho = rcp_nr(rho);
+    const Real u = m.mx * inv_rho + Real(0.5) * a.dpdx * inv_rho;
+    const Real v = m.my * inv_rho, w = m.mz * inv_rho;
+    if (a.mac_all || (fl & F_KEEPMAC)) {
+        a.rho_n[ic] = rho;
+        a.u_n[ic] = u;
+        a.v_n[ic] = v;
+        a.w_n[ic] = w;
+    }
+    if (a.aux_all || (fl & F_NEISOLID)) {
+        a.Fwx[ic] = Fx;
+        a.Fwy[ic] = Fy;
+        a.Fwz[ic] = Fz;
+    }
+    if (a.aux_all) {
+        a.Gx[ic] = Real(0);
+        a.Gy[ic] = Real(0);
+        a.Gz[ic] = Real(0);
+    }
+
+    const Real tau0 = rcp(omega);
+    Real tauSGS;
+    if constexpr (SGS == 0) {
+        tauSGS = wale_tau<Real>(a, ic, i, j, k, u, v, w);
+    } else if constexpr (SGS == 1) {
+        const Real sdf = (fl & F_HASSDF) ? a.sdf[ic] : Real(10000);
+        tauSGS = smagorinsky_tau<Real>(m, u, v, w, tau0, inv_rho, a.LES, sdf, a.Cs, a.damping);
+    } else {
+        tauSGS = Real(0);
+    }
+    // the interpolated bounce-back of the next step reads this cell's tauSGS
+    if (a.aux_all || (fl & F_NEISOLID)) a.tau[ic] = tauSGS;
+
+    const Real tau = sponge_tau<Real>(a, tau0, tauSGS, i, j, k);
+    const Real omegaEff = rcp(tau + tauSGS);
+    cumulant27<Real>(ft, rho, u, v, w, omegaEff, a.omegaB, a.clim);
+    const uint32_t doff = icu * (uint32_t)sizeof(S);
+    if (a.tune & TUNE_NT_STORE) {
+#pragma unroll
+        for (int q = 0; q < 27; q++)
+            Codec<S, Real, 27>::store_nt((S *)((char *)(fd + (size_t)q * N) + doff), ft[q], q, w27<Real>(q));
+    } else {
+#pragma unroll
+        for (int q = 0; q < 27; q++)
+            Codec<S, Real, 27>::store((S *)((char *)(fd + (size_t)q * N) + doff), ft[q], q, w27<Real>(q));
+    }
+}
+
+}  // namespace lbffs

@@ -11,6 +11,7 @@
 
 #include "../../include/lbffs.h"
 #include "d3q19_step.hpp"
+#include "d3q27_step.hpp"
 
 using namespace lbffs;
 
@@ -38,6 +39,7 @@ enum : int {
     PACK_WALLFN_ZMAX = 4,      // WallFunction on the nzMax face: the reference samples k+1 there
     PACK_2D_ZCODE = 8,         // nz==1 with a z-face condition that samples the layer k=1
     PACK_BAD_BOUNDARY = 16,    // boundary value other than 0/1
+    PACK_NEISOLID_FACE = 32,   // D3Q27: neiSolid on a wall face (the reference reads solid[-1] there)
 };
 
 }  // namespace
@@ -79,7 +81,7 @@ constexpr int kRedThreads = 256;
 // ---------------------------------------------------------------- kernels
 
 template <typename Real>
-__global__ void pack_flags_kernel(uint32_t N, int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
+__global__ void pack_flags_kernel(int Q, uint32_t N, int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
                                   const int32_t *boundary, const int32_t *b1, const int32_t *b2, const int32_t *b3,
                                   const uint8_t *solid, const uint8_t *neiSolid, const Real *sdf, const Real *omega,
                                   Real omega0, const Real *w0, uint16_t *flag, int *err)
@@ -110,7 +112,23 @@ __global__ void pack_flags_kernel(uint32_t N, int nx, int ny, int nz, FastDiv dn
         }
     }
     if (neiSolid[ic] == 1) fl |= F_NEISOLID;
-    const bool keep = (i <= 1 || i >= nx - 2 || j <= 1 || j >= ny - 2 || (nz > 1 && (k <= 1 || k >= nz - 2)));
+    bool keep = (i <= 1 || i >= nx - 2 || j <= 1 || j >= ny - 2 || (nz > 1 && (k <= 1 || k >= nz - 2)));
+    if (Q == 27) {
+        // a NEISOLID cell tests solid[upID[q]] for all 26 links; on a wall face one of
+        // them is -1 in the reference
+        const bool on_wall = ((i == 0 || i == nx - 1) && is_wall_code(c1)) || ((j == 0 || j == ny - 1) && is_wall_code(c2)) ||
+                             ((k == 0 || k == nz - 1) && is_wall_code(c3));
+        if (!sol && bnd == 1 && neiSolid[ic] == 1 && on_wall) e |= PACK_NEISOLID_FACE;
+        // the interpolated bounce-back reads the previous-step moments of a NEISOLID cell
+        // and of its neighbours: keep them stored every step
+        for (int dz = -1; dz <= 1 && !keep; dz++)
+            for (int dy = -1; dy <= 1 && !keep; dy++)
+                for (int dx = -1; dx <= 1 && !keep; dx++) {
+                    const int ii = i + dx, jj = j + dy, kz = k + dz;
+                    if (ii < 0 || ii >= nx || jj < 0 || jj >= ny || kz < 0 || kz >= nz) continue;
+                    if (neiSolid[(long)ii + (long)nx * ((long)jj + (long)ny * kz)] == 1) keep = true;
+                }
+    }
     if (keep) fl |= F_KEEPMAC;
     if (sdf[ic] != Real(10000)) fl |= F_HASSDF;
     if (omega[ic] != omega0) fl |= F_OMEGAVAR;
@@ -126,7 +144,7 @@ __global__ void encode_kernel(uint32_t N, const Real *src, S *dst)
     if (ic >= N) return;
 #pragma unroll
     for (int q = 0; q < Q; q++)
-        Codec<S, Real, Q>::store(dst + (size_t)q * N + ic, src[(size_t)q * N + ic], q, w19<Real>(q));
+        Codec<S, Real, Q>::store(dst + (size_t)q * N + ic, src[(size_t)q * N + ic], q, wq<Real, Q>(q));
 }
 
 template <typename S, typename Real, int Q>
@@ -136,7 +154,7 @@ __global__ void decode_kernel(uint32_t N, const S *src, Real *dst)
     if (ic >= N) return;
 #pragma unroll
     for (int q = 0; q < Q; q++)
-        dst[(size_t)q * N + ic] = Codec<S, Real, Q>::load(src + (size_t)q * N + ic, q, w19<Real>(q));
+        dst[(size_t)q * N + ic] = Codec<S, Real, Q>::load(src + (size_t)q * N + ic, q, wq<Real, Q>(q));
 }
 
 __device__ inline double wave_sum(double x)
@@ -147,7 +165,7 @@ __device__ inline double wave_sum(double x)
 }
 
 // pass 1: fixed grid, grid-stride over cells, per-block partial sums (deterministic order)
-template <typename S, typename Real>
+template <typename S, typename Real, int Q>
 __global__ void __launch_bounds__(kRedThreads) reduce_kernel(uint32_t N, const uint16_t *flag, const S *f,
                                                              const Real *Fwx, const Real *Fwy, const Real *Fwz,
                                                              const Real *Gx, const Real *Gy, const Real *Gz,
@@ -164,7 +182,7 @@ __global__ void __launch_bounds__(kRedThreads) reduce_kernel(uint32_t N, const u
         acc[5] += (double)Gz[ic];
         Real rho = 0;
 #pragma unroll
-        for (int q = 0; q < 19; q++) rho += Codec<S, Real, 19>::load(f + (size_t)q * N + ic, q, w19<Real>(q));
+        for (int q = 0; q < Q; q++) rho += Codec<S, Real, Q>::load(f + (size_t)q * N + ic, q, wq<Real, Q>(q));
         acc[6] += (double)rho - 1.0;
         acc[7] += 1.0;
     }
@@ -277,7 +295,7 @@ int pack_flags(lbffs_ctx *c)
     const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
     HIP_TRY(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
     Real om0 = (Real)c->d.omega0;
-    hipLaunchKernelGGL((pack_flags_kernel<Real>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL((pack_flags_kernel<Real>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream, c->d.q,
                        (uint32_t)c->N, nx, ny, nz, make_fastdiv(nx), make_fastdiv(ny), c->boundary, c->b1, c->b2,
                        c->b3, c->solid, c->neiSolid, (const Real *)c->sdf, (const Real *)c->omega, om0,
                        (const Real *)c->w0, c->flag, c->d_err);
@@ -306,6 +324,7 @@ int pack_flags(lbffs_ctx *c)
         if (err & PACK_FACE_NOT_FLAGGED) m += " fluid cell on a lattice face with boundary==0;";
         if (err & PACK_WALLFN_ZMAX) m += " WallFunction on the nzMax face (the reference samples k+1 there);";
         if (err & PACK_2D_ZCODE) m += " nz==1 with an Outlet/FixedDensity/inflowing FixedVelocity z face;";
+        if (err & PACK_NEISOLID_FACE) m += " neiSolid on a wall face (D3Q27 reads outside the lattice there);";
         return fail(LBFFS_E_INVALID, m);
     }
     c->sponge_mask = 0;
@@ -315,8 +334,8 @@ int pack_flags(lbffs_ctx *c)
     return 0;
 }
 
-template <typename S, typename Real>
-void set_args(lbffs_ctx *c, Step19Args<Real> &a, const lbffs_step_params *p)
+template <typename S, typename Real, int Q>
+void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
 {
     const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
     a.N = (uint32_t)c->N;
@@ -325,9 +344,12 @@ void set_args(lbffs_ctx *c, Step19Args<Real> &a, const lbffs_step_params *p)
     a.nz = nz;
     a.dnx = make_fastdiv(nx);
     a.dny = make_fastdiv(ny);
-    for (int q = 0; q < 19; q++) {
-        const int zoff = (nz == 1) ? 0 : L19::cz[q];  // nz==1: z links stay in the layer (kernel.cl:907-989)
-        a.off[q] = L19::cx[q] + nx * L19::cy[q] + nx * ny * zoff;
+    for (int q = 0; q < Q; q++) {
+        const int cx = Q == 27 ? L27::cx[q] : L19::cx[q];
+        const int cy = Q == 27 ? L27::cy[q] : L19::cy[q];
+        const int cz = Q == 27 ? L27::cz[q] : L19::cz[q];
+        const int zoff = (nz == 1) ? 0 : cz;  // nz==1: z links stay in the layer (K19:907-989, K27:797-820)
+        a.off[q] = cx + nx * cy + nx * ny * zoff;
     }
     a.flag = c->flag;
     a.u0 = (const Real *)c->u0;
@@ -347,6 +369,7 @@ void set_args(lbffs_ctx *c, Step19Args<Real> &a, const lbffs_step_params *p)
     a.omegaB = (Real)p->omegaB;
     a.spz = (Real)p->spzWidth;
     a.Cs = (Real)p->Cs;
+    a.clim = (Real)p->clim;
     a.omega0 = (Real)c->d.omega0;
     a.damping = p->damping;
     a.sponge_mask = c->sponge_mask;
@@ -357,11 +380,20 @@ void set_args(lbffs_ctx *c, Step19Args<Real> &a, const lbffs_step_params *p)
     a.tune = c->tune;
 }
 
-template <typename S, typename Real>
-int run_steps(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
+template <typename S, typename Real, int Q, int SGS>
+void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream_t st)
 {
-    Step19Args<Real> a;
-    set_args<S, Real>(c, a, p);
+    if constexpr (Q == 27)
+        hipLaunchKernelGGL((step27_kernel<S, Real, SGS>), dim3(grid), dim3(block), 0, st, a);
+    else
+        hipLaunchKernelGGL((step19_kernel<S, Real, SGS>), dim3(grid), dim3(block), 0, st, a);
+}
+
+template <typename S, typename Real, int Q>
+int run_steps_q(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
+{
+    StepArgs<Real, Q> a;
+    set_args<S, Real, Q>(c, a, p);
     const int block = c->block;
     const unsigned grid = grid_for(c->N, block);
     for (int s = 0; s < nsteps; s++) {
@@ -379,21 +411,21 @@ int run_steps(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
         a.aux_all = (p->store_every || last) ? 1 : 0;
         a.mac_all = (a.aux_all || c->d.sgs == LBFFS_SGS_WALE) ? 1 : 0;
         switch (c->d.sgs) {
-        case LBFFS_SGS_WALE:
-            hipLaunchKernelGGL((step19_kernel<S, Real, 0>), dim3(grid), dim3(block), 0, c->stream, a);
-            break;
-        case LBFFS_SGS_SMAGORINSKY:
-            hipLaunchKernelGGL((step19_kernel<S, Real, 1>), dim3(grid), dim3(block), 0, c->stream, a);
-            break;
-        default:
-            hipLaunchKernelGGL((step19_kernel<S, Real, 2>), dim3(grid), dim3(block), 0, c->stream, a);
-            break;
+        case LBFFS_SGS_WALE: launch_step<S, Real, Q, 0>(a, grid, block, c->stream); break;
+        case LBFFS_SGS_SMAGORINSKY: launch_step<S, Real, Q, 1>(a, grid, block, c->stream); break;
+        default: launch_step<S, Real, Q, 2>(a, grid, block, c->stream); break;
         }
         HIP_TRY(hipGetLastError());
         c->cur ^= 1;
         c->mcur ^= 1;
     }
     return 0;
+}
+
+template <typename S, typename Real>
+int run_steps(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
+{
+    return c->d.q == 27 ? run_steps_q<S, Real, 27>(c, nsteps, p) : run_steps_q<S, Real, 19>(c, nsteps, p);
 }
 
 template <typename S, typename Real>
@@ -409,9 +441,14 @@ int upload_f(lbffs_ctx *c, const void *host)
     if (int e = dalloc(&tmp, n * sizeof(Real))) return e;
     hipError_t e1 = hipMemcpy(tmp, host, n * sizeof(Real), hipMemcpyHostToDevice);
     if (e1 == hipSuccess) {
-        for (int b = 0; b < 2; b++)
-            hipLaunchKernelGGL((encode_kernel<S, Real, 19>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
-                               (uint32_t)c->N, (const Real *)tmp, (S *)c->f[b]);
+        for (int b = 0; b < 2; b++) {
+            if (c->d.q == 27)
+                hipLaunchKernelGGL((encode_kernel<S, Real, 27>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
+                                   (uint32_t)c->N, (const Real *)tmp, (S *)c->f[b]);
+            else
+                hipLaunchKernelGGL((encode_kernel<S, Real, 19>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
+                                   (uint32_t)c->N, (const Real *)tmp, (S *)c->f[b]);
+        }
         e1 = hipGetLastError();
         if (e1 == hipSuccess) e1 = hipStreamSynchronize(c->stream);
     }
@@ -431,8 +468,12 @@ int download_f(lbffs_ctx *c, void *host)
     }
     void *tmp = nullptr;
     if (int e = dalloc(&tmp, n * sizeof(Real))) return e;
-    hipLaunchKernelGGL((decode_kernel<S, Real, 19>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
-                       (uint32_t)c->N, (const S *)c->f[c->cur], (Real *)tmp);
+    if (c->d.q == 27)
+        hipLaunchKernelGGL((decode_kernel<S, Real, 27>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
+                           (uint32_t)c->N, (const S *)c->f[c->cur], (Real *)tmp);
+    else
+        hipLaunchKernelGGL((decode_kernel<S, Real, 19>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
+                           (uint32_t)c->N, (const S *)c->f[c->cur], (Real *)tmp);
     hipError_t e1 = hipGetLastError();
     if (e1 == hipSuccess) e1 = hipStreamSynchronize(c->stream);
     if (e1 == hipSuccess) e1 = hipMemcpy(host, tmp, n * sizeof(Real), hipMemcpyDeviceToHost);
@@ -444,10 +485,16 @@ int download_f(lbffs_ctx *c, void *host)
 template <typename S, typename Real>
 int reduce_impl(lbffs_ctx *c, double *out)
 {
-    hipLaunchKernelGGL((reduce_kernel<S, Real>), dim3(kRedBlocks), dim3(kRedThreads), 0, c->stream, (uint32_t)c->N,
-                       c->flag, (const S *)c->f[c->cur], (const Real *)c->Fwx, (const Real *)c->Fwy,
-                       (const Real *)c->Fwz, (const Real *)c->Gx, (const Real *)c->Gy, (const Real *)c->Gz,
-                       c->d_part);
+    if (c->d.q == 27)
+        hipLaunchKernelGGL((reduce_kernel<S, Real, 27>), dim3(kRedBlocks), dim3(kRedThreads), 0, c->stream,
+                           (uint32_t)c->N, c->flag, (const S *)c->f[c->cur], (const Real *)c->Fwx,
+                           (const Real *)c->Fwy, (const Real *)c->Fwz, (const Real *)c->Gx, (const Real *)c->Gy,
+                           (const Real *)c->Gz, c->d_part);
+    else
+        hipLaunchKernelGGL((reduce_kernel<S, Real, 19>), dim3(kRedBlocks), dim3(kRedThreads), 0, c->stream,
+                           (uint32_t)c->N, c->flag, (const S *)c->f[c->cur], (const Real *)c->Fwx,
+                           (const Real *)c->Fwy, (const Real *)c->Fwz, (const Real *)c->Gx, (const Real *)c->Gy,
+                           (const Real *)c->Gz, c->d_part);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(64), 0, c->stream, c->d_part, kRedBlocks, c->d_red);
     HIP_TRY(hipGetLastError());
@@ -507,13 +554,16 @@ int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out)
     if (!desc || !out) return fail(LBFFS_E_INVALID, "null argument");
     *out = nullptr;
     if (desc->abi_version != LBFFS_ABI_VERSION) return fail(LBFFS_E_INVALID, "abi_version mismatch");
-    if (desc->q != 19) return fail(LBFFS_E_UNSUPPORTED, "only q=19 is implemented in this build");
+    if (desc->q != 19 && desc->q != 27) return fail(LBFFS_E_UNSUPPORTED, "q must be 19 or 27");
     if (desc->nx < 2 || desc->ny < 2 || desc->nz < 1) return fail(LBFFS_E_INVALID, "lattice needs nx,ny >= 2, nz >= 1");
     const long N = (long)desc->nx * desc->ny * desc->nz;
     if (N >= (1L << 31) / 1 || (long)desc->q * N < 0) return fail(LBFFS_E_INVALID, "cell count must be below 2^31");
     if (desc->storage < 0 || desc->storage > 3) return fail(LBFFS_E_INVALID, "storage");
     if (desc->sgs < 0 || desc->sgs > 2) return fail(LBFFS_E_INVALID, "sgs");
-    if (desc->collision != LBFFS_COLL_RR) return fail(LBFFS_E_UNSUPPORTED, "D3Q19 uses the RR collision");
+    if (desc->q == 19 && desc->collision != LBFFS_COLL_RR)
+        return fail(LBFFS_E_UNSUPPORTED, "D3Q19 runs the recursive-regularized collision (K19:3637)");
+    if (desc->q == 27 && desc->collision != LBFFS_COLL_CUMULANT)
+        return fail(LBFFS_E_UNSUPPORTED, "D3Q27 runs the cumulant collision (K27:2824)");
     if (desc->block != 0 && (desc->block % 64 != 0 || desc->block > 256 || desc->block < 64))
         return fail(LBFFS_E_INVALID, "block must be 64, 128, 192 or 256");
 

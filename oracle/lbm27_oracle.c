@@ -26,11 +26,19 @@ static int c2q27(int cx, int cy, int cz)
 /* K27:48-160 reflectQ */
 int orc27_reflect(int q) { return c2q27(-ORC27_CX[q], -ORC27_CY[q], -ORC27_CZ[q]); }
 
-/* K27:175-527 mirrorQx / mirrorQy / mirrorQz: flip one component */
+/* K27:186-528 mirrorQx / mirrorQy / mirrorQz (the tables, not the unused mirrorQ*s
+ * variants): a direction with a component along the axis has that component flipped;
+ * one without (parallel to the mirror plane) is reflected whole, e.g. mirrorQx(3) = 4. */
 int orc27_mirror(int q, int axis)
 {
     int c[3] = {ORC27_CX[q], ORC27_CY[q], ORC27_CZ[q]};
-    c[axis] = -c[axis];
+    if (c[axis] != 0) {
+        c[axis] = -c[axis];
+    } else {
+        c[0] = -c[0];
+        c[1] = -c[1];
+        c[2] = -c[2];
+    }
     return c2q27(c[0], c[1], c[2]);
 }
 

@@ -14,8 +14,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
+ORACLE_FMA_SO = os.path.join(HERE, "liboracle_fma.so")
 REFRUN_SO = os.path.join(HERE, "librefrun.so")
 REF_CO19 = os.path.join(HERE, "_ref", "kernel19.co")
+REF_CO27 = os.path.join(HERE, "_ref", "kernel27.co")
 REF_HOST_SO = os.path.join(HERE, "_ref", "libref_host.so")
 
 JACOBI, LEX_GS = 0, 1
@@ -30,36 +32,47 @@ class OrcParams(C.Structure):
 
 
 _orc = None
+_orc_fma = None
 _ref = None
 _refhost = None
 
 
-def orc():
-    global _orc
+def orc(fma=False):
+    """The oracle library; fma=True gives the same sources built with floating-point
+    contraction (fused multiply-add), the arithmetic the OpenCL and HIP compilers use."""
+    global _orc, _orc_fma
+    if fma:
+        if _orc_fma is None:
+            _orc_fma = _bind(C.CDLL(ORACLE_FMA_SO))
+        return _orc_fma
     if _orc is None:
-        L = C.CDLL(ORACLE_SO)
-        vp = C.c_void_p
-        for p in ("f32", "f64"):
-            for lat in ("19", "27"):
-                getattr(L, f"orc{lat}_step_{p}").argtypes = [C.POINTER(OrcParams)] + [vp] * 23
-                getattr(L, f"orc{lat}_step_{p}").restype = C.c_int
-        L.orc27_collide_f32.argtypes = [vp] + [C.c_float] * 8 + [vp]
-        L.orc27_collide_f64.argtypes = [vp] + [C.c_double] * 8 + [vp]
-        L.orc27_upwind_B.argtypes = [C.c_int] * 10
-        L.orc27_upwind_B.restype = C.c_long
-        L.orc27_mirror.argtypes = [C.c_int, C.c_int]
-        L.orc19_collide_f32.argtypes = [C.c_int, vp] + [C.c_float] * 8 + [vp]
-        L.orc19_collide_f64.argtypes = [C.c_int, vp] + [C.c_double] * 8 + [vp]
-        L.orc19_smagorinsky_f32.argtypes = [vp] + [C.c_float] * 8 + [C.c_int]
-        L.orc19_smagorinsky_f32.restype = C.c_float
-        L.orc19_smagorinsky_f64.argtypes = [vp] + [C.c_double] * 8 + [C.c_int]
-        L.orc19_smagorinsky_f64.restype = C.c_double
-        L.orc_upwind_B.argtypes = [C.c_int] * 10
-        L.orc_upwind_B.restype = C.c_long
-        L.orc_reflect_or_mirror.argtypes = [C.c_int] * 4
-        L.orc_corner.argtypes = [C.c_int] * 7
-        _orc = L
+        _orc = _bind(C.CDLL(ORACLE_SO))
     return _orc
+
+
+def _bind(L):
+    """Declare the argument types of the oracle entry points."""
+    vp = C.c_void_p
+    for p in ("f32", "f64"):
+        for lat in ("19", "27"):
+            getattr(L, f"orc{lat}_step_{p}").argtypes = [C.POINTER(OrcParams)] + [vp] * 23
+            getattr(L, f"orc{lat}_step_{p}").restype = C.c_int
+    L.orc27_collide_f32.argtypes = [vp] + [C.c_float] * 8 + [vp]
+    L.orc27_collide_f64.argtypes = [vp] + [C.c_double] * 8 + [vp]
+    L.orc27_upwind_B.argtypes = [C.c_int] * 10
+    L.orc27_upwind_B.restype = C.c_long
+    L.orc27_mirror.argtypes = [C.c_int, C.c_int]
+    L.orc19_collide_f32.argtypes = [C.c_int, vp] + [C.c_float] * 8 + [vp]
+    L.orc19_collide_f64.argtypes = [C.c_int, vp] + [C.c_double] * 8 + [vp]
+    L.orc19_smagorinsky_f32.argtypes = [vp] + [C.c_float] * 8 + [C.c_int]
+    L.orc19_smagorinsky_f32.restype = C.c_float
+    L.orc19_smagorinsky_f64.argtypes = [vp] + [C.c_double] * 8 + [C.c_int]
+    L.orc19_smagorinsky_f64.restype = C.c_double
+    L.orc_upwind_B.argtypes = [C.c_int] * 10
+    L.orc_upwind_B.restype = C.c_long
+    L.orc_reflect_or_mirror.argtypes = [C.c_int] * 4
+    L.orc_corner.argtypes = [C.c_int] * 7
+    return L
 
 
 def _p(a):
@@ -78,7 +91,7 @@ def initial_state(case, real=np.float32):
     return st
 
 
-def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=None, sgs=None):
+def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=None, sgs=None, fma=False):
     """Run nsteps of the restated reference step (D3Q19, or D3Q27 when the case has
     q == 27).  Returns the state dict (f = current populations after the swaps, plus the
     macro arrays)."""
@@ -95,7 +108,7 @@ def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=
     u8 = {k: np.ascontiguousarray(a[k], np.uint8) for k in ("solid", "neiSolid")}
     f = st["f"]
     ft = f.copy()  # initialization.hpp: fTmp = f
-    fn = getattr(orc(), f"orc{getattr(case, 'q', 19)}_step_{precision}")
+    fn = getattr(orc(fma), f"orc{getattr(case, 'q', 19)}_step_{precision}")
     for _ in range(nsteps):
         rc = fn(C.byref(P), _p(f), _p(ft), _p(ints["boundary"]), _p(ints["boundary1"]), _p(ints["boundary2"]),
                 _p(ints["boundary3"]), _p(sides["sdf"]), _p(u8["solid"]), _p(u8["neiSolid"]), _p(sides["u0"]),
@@ -145,19 +158,30 @@ def refrun():
         L.refrun_step19.argtypes = ([C.c_char_p] + [C.c_int] * 5 + [vp] * 23 + [C.c_float] * 3 + [C.c_int]
                                     + [C.c_float] * 2)
         L.refrun_step19.restype = C.c_int
+        L.refrun_step27.argtypes = L.refrun_step19.argtypes + [C.c_float]
+        L.refrun_step27.restype = C.c_int
         L.refrun_error.restype = C.c_char_p
         _ref = L
     return _ref
 
 
-def run_reference_gpu(case, nsteps, block=256, rho_av=1.0, LES=None):
-    """Run the reference kernel.cl (compiled for gfx950) for nsteps with its own host loop."""
-    if not os.path.exists(REF_CO19):
-        raise FileNotFoundError(REF_CO19 + " (make -C oracle ref)")
+def run_reference_gpu(case, nsteps, block=None, rho_av=1.0, LES=None, state=None):
+    """Run the reference kernel.cl of the case's lattice (compiled for gfx950) for nsteps
+    with its own host loop, from the case's initial populations and zero macros, or from
+    `state` (a dict of f and the macro arrays, as run19 returns)."""
+    co = REF_CO27 if case.q == 27 else REF_CO19
+    if block is None:  # the runner launches exactly N work-items: the largest group dividing N
+        block = next(b for b in (256, 192, 128, 64, 32, 16, 8, 4, 2, 1) if case.N % b == 0)
+    if not os.path.exists(co):
+        raise FileNotFoundError(co + " (make -C oracle ref)")
     a = case.arrays
     N = case.N
-    st = {k: np.zeros(N, np.float32) for k in MACRO}
-    f = np.ascontiguousarray(a["f"], np.float32).copy()
+    if state is None:
+        st = {k: np.zeros(N, np.float32) for k in MACRO}
+        f = np.ascontiguousarray(a["f"], np.float32).copy()
+    else:
+        st = {k: np.ascontiguousarray(state[k], np.float32).copy() for k in MACRO}
+        f = np.ascontiguousarray(state["f"], np.float32).copy()
     p = case.params
     typed = [f, np.ascontiguousarray(a["boundary"], np.int32), np.ascontiguousarray(a["boundary1"], np.int32),
              np.ascontiguousarray(a["boundary2"], np.int32), np.ascontiguousarray(a["boundary3"], np.int32),
@@ -166,13 +190,32 @@ def run_reference_gpu(case, nsteps, block=256, rho_av=1.0, LES=None):
              np.ascontiguousarray(a["v0"], np.float32), np.ascontiguousarray(a["w0"], np.float32),
              st["Fwx"], st["Fwy"], st["Fwz"], st["tauSGS"], st["rho"], st["u"], st["v"], st["w"],
              np.ascontiguousarray(a["omega"], np.float32), st["GxIBM"], st["GyIBM"], st["GzIBM"]]
-    rc = refrun().refrun_step19(REF_CO19.encode(), case.nx, case.ny, case.nz, nsteps, block,
-                                *[_p(t) for t in typed],
-                                p.get("dpdx", 0.0), rho_av, p.get("LES", 0.0) if LES is None else LES, 0,
-                                p.get("omegaB", 0.1), p.get("spzWidth", 0.1))
+    args = [co.encode(), case.nx, case.ny, case.nz, nsteps, block, *[_p(t) for t in typed],
+            p.get("dpdx", 0.0), rho_av, p.get("LES", 0.0) if LES is None else LES, 0,
+            p.get("omegaB", 0.1), p.get("spzWidth", 0.1)]
+    if case.q == 27:
+        rc = refrun().refrun_step27(*args, p.get("clim", 0.01))
+    else:
+        rc = refrun().refrun_step19(*args)
     if rc != 0:
         raise RuntimeError("reference run failed: " + refrun().refrun_error().decode())
     st["f"] = f
+    return st
+
+
+def race_free_state(case, warm, iters=16):
+    """A developed FP64 state whose stored rho/u/v/w are the fixed point of one step:
+    f after `warm` oracle steps, then the macros replaced by those the next step computes
+    from that f until they stop changing.  From such a state every neighbour read of the
+    macros returns the same value whether or not the neighbour has already run its step
+    in the same launch, so one launch of the reference kernel is deterministic and can be
+    compared value by value with the oracle."""
+    st = run19(case, warm, precision="f64")
+    for _ in range(iters):
+        nxt = run19(case, 1, precision="f64", state=st)
+        st = {k: v.copy() for k, v in st.items()}
+        for m in ("rho", "u", "v", "w"):
+            st[m] = nxt[m]
     return st
 
 

@@ -1,11 +1,12 @@
-// ref_runner.cpp -- runs the reference's own k_streamingCollision (D3Q19_opencl/kernel.cl,
-// built by `make -C oracle ref` into oracle/_ref/kernel19.co) on the GPU with the
-// reference host loop: launch over nx*ny*nz work-items, wait, swap f/fTmp
-// (ns.cpp:509-603, without output and moving walls).
+// ref_runner.cpp -- runs the reference's own k_streamingCollision (D3Q19_opencl/kernel.cl and
+// D3Q27_opencl/kernel.cl, built by `make -C oracle ref` into oracle/_ref/kernel19.co and
+// kernel27.co) on the GPU with the reference host loop: launch over nx*ny*nz work-items,
+// wait, swap f/fTmp (D3Q19 ns.cpp:509-603, D3Q27 ns.cpp:540-640, without output and moving
+// walls).
 //
 // TEST INFRASTRUCTURE ONLY: the checker that pins the CPU oracle and the HIP product to
-// the reference kernel on the same inputs.  The reference kernel has no bounds guard
-// (kernel.cl:3548) and a few branches that index outside the lattice; refrun_step19
+// the reference kernel on the same inputs.  The reference kernels have no bounds guard
+// (K19:3548, K27:2737) and a few branches that index outside the lattice; the runner
 // refuses inputs that would reach them and launches exactly N work-items.
 #include <hip/hip_runtime.h>
 
@@ -43,18 +44,19 @@ extern "C" {
 
 const char *refrun_error(void) { return g_err.c_str(); }
 
+namespace {
 // Arrays are host arrays in the reference layout.  In/out arrays are overwritten with
-// the state after nsteps: f (19N, current populations after the swaps), Fw*, tauSGS,
-// rho/u/v/w, G*.
-int refrun_step19(const char *co_path, int nx, int ny, int nz, int nsteps, int block, float *f,
-                  const int32_t *boundary, const int32_t *b1, const int32_t *b2, const int32_t *b3, const float *sdf,
-                  const uint8_t *solid, const uint8_t *neiSolid, const float *u0, const float *v0, const float *w0,
-                  float *Fwx, float *Fwy, float *Fwz, float *tauSGS, float *rho, float *u, float *v, float *w,
-                  const float *omega, float *Gx, float *Gy, float *Gz, float dpdx, float rho_av, float LES,
-                  int isReadMovingWalls, float omegaB, float spzWidth)
+// the state after nsteps: f (Q*N, current populations after the swaps), Fw*, tauSGS,
+// rho/u/v/w, G*.  Q = 27 appends the limiter clim to the kernel's arguments (K27:2731).
+int run_steps(int Q, const char *co_path, int nx, int ny, int nz, int nsteps, int block, float *f,
+              const int32_t *boundary, const int32_t *b1, const int32_t *b2, const int32_t *b3, const float *sdf,
+              const uint8_t *solid, const uint8_t *neiSolid, const float *u0, const float *v0, const float *w0,
+              float *Fwx, float *Fwy, float *Fwz, float *tauSGS, float *rho, float *u, float *v, float *w,
+              const float *omega, float *Gx, float *Gy, float *Gz, float dpdx, float rho_av, float LES,
+              int isReadMovingWalls, float omegaB, float spzWidth, float clim)
 {
     const long N = (long)nx * ny * nz;
-    if (nx < 2 || ny < 2 || nz < 1 || N <= 0 || 19 * N >= (1L << 31)) return err("lattice size");
+    if (nx < 2 || ny < 2 || nz < 1 || N <= 0 || (long)Q * N >= (1L << 31)) return err("lattice size");
     if (block <= 0 || block > 1024 || N % block != 0) return err("N must be a multiple of the work-group size");
     // inputs that would send the reference kernel outside its buffers
     for (long ic = 0; ic < N; ic++) {
@@ -82,7 +84,7 @@ int refrun_step19(const char *co_path, int nx, int ny, int nz, int nsteps, int b
         Buf *b;
         const void *src;
         size_t bytes;
-    } up[] = {{&F, f, 19 * nf},     {&FT, f, 19 * nf},   {&B, boundary, ni}, {&B1, b1, ni},     {&B2, b2, ni},
+    } up[] = {{&F, f, Q * nf},      {&FT, f, Q * nf},   {&B, boundary, ni}, {&B1, b1, ni},     {&B2, b2, ni},
               {&B3, b3, ni},        {&SDF, sdf, nf},     {&SOL, solid, (size_t)N}, {&NEI, neiSolid, (size_t)N},
               {&U0, u0, nf},        {&V0, v0, nf},       {&W0, w0, nf},      {&FX, Fwx, nf},    {&FY, Fwy, nf},
               {&FZ, Fwz, nf},       {&TAU, tauSGS, nf},  {&RHO, rho, nf},    {&U, u, nf},       {&V, v, nf},
@@ -105,9 +107,11 @@ int refrun_step19(const char *co_path, int nx, int ny, int nz, int nsteps, int b
             void *params[] = {&fsrc, &fdst, &B.p, &B1.p, &B2.p, &B3.p, &SDF.p, &SOL.p, &NEI.p, &U0.p, &V0.p, &W0.p,
                               &FX.p, &FY.p, &FZ.p, &TAU.p, &RHO.p, &U.p, &V.p, &W.p, &OM.p, &GX.p, &GY.p, &GZ.p,
                               &elements, &dpdx, &rho_av, &nx, &ny, &nz, &LES, &isReadMovingWalls, &omegaB,
-                              &spzWidth};
+                              &spzWidth, &clim};
+            const int nargs = (int)(sizeof(params) / sizeof(params[0])) - (Q == 27 ? 0 : 1);
+            std::vector<void *> args(params, params + nargs);
             hipError_t e = hipModuleLaunchKernel(fn, (unsigned)(N / block), 1, 1, (unsigned)block, 1, 1, 0, nullptr,
-                                                 params, nullptr);
+                                                 args.data(), nullptr);
             if (e == hipSuccess) e = hipDeviceSynchronize();  // queue.finish(), ns.cpp:537
             if (e != hipSuccess) rc = err(std::string("launch: ") + hipGetErrorString(e));
             std::swap(fsrc, fdst);  // ns.cpp:602
@@ -116,7 +120,7 @@ int refrun_step19(const char *co_path, int nx, int ny, int nz, int nsteps, int b
             void *dst;
             const void *src;
             size_t bytes;
-        } down[] = {{f, fsrc, 19 * nf}, {Fwx, FX.p, nf}, {Fwy, FY.p, nf}, {Fwz, FZ.p, nf}, {tauSGS, TAU.p, nf},
+        } down[] = {{f, fsrc, Q * nf}, {Fwx, FX.p, nf}, {Fwy, FY.p, nf}, {Fwz, FZ.p, nf}, {tauSGS, TAU.p, nf},
                     {rho, RHO.p, nf},   {u, U.p, nf},    {v, V.p, nf},    {w, W.p, nf},    {Gx, GX.p, nf},
                     {Gy, GY.p, nf},     {Gz, GZ.p, nf}};
         for (auto &x : down)
@@ -124,6 +128,31 @@ int refrun_step19(const char *co_path, int nx, int ny, int nz, int nsteps, int b
     }
     (void)hipModuleUnload(mod);
     return rc;
+}
+}  // namespace
+
+int refrun_step19(const char *co_path, int nx, int ny, int nz, int nsteps, int block, float *f,
+                  const int32_t *boundary, const int32_t *b1, const int32_t *b2, const int32_t *b3, const float *sdf,
+                  const uint8_t *solid, const uint8_t *neiSolid, const float *u0, const float *v0, const float *w0,
+                  float *Fwx, float *Fwy, float *Fwz, float *tauSGS, float *rho, float *u, float *v, float *w,
+                  const float *omega, float *Gx, float *Gy, float *Gz, float dpdx, float rho_av, float LES,
+                  int isReadMovingWalls, float omegaB, float spzWidth)
+{
+    return run_steps(19, co_path, nx, ny, nz, nsteps, block, f, boundary, b1, b2, b3, sdf, solid, neiSolid, u0, v0,
+                     w0, Fwx, Fwy, Fwz, tauSGS, rho, u, v, w, omega, Gx, Gy, Gz, dpdx, rho_av, LES,
+                     isReadMovingWalls, omegaB, spzWidth, 0.f);
+}
+
+int refrun_step27(const char *co_path, int nx, int ny, int nz, int nsteps, int block, float *f,
+                  const int32_t *boundary, const int32_t *b1, const int32_t *b2, const int32_t *b3, const float *sdf,
+                  const uint8_t *solid, const uint8_t *neiSolid, const float *u0, const float *v0, const float *w0,
+                  float *Fwx, float *Fwy, float *Fwz, float *tauSGS, float *rho, float *u, float *v, float *w,
+                  const float *omega, float *Gx, float *Gy, float *Gz, float dpdx, float rho_av, float LES,
+                  int isReadMovingWalls, float omegaB, float spzWidth, float clim)
+{
+    return run_steps(27, co_path, nx, ny, nz, nsteps, block, f, boundary, b1, b2, b3, sdf, solid, neiSolid, u0, v0,
+                     w0, Fwx, Fwy, Fwz, tauSGS, rho, u, v, w, omega, Gx, Gy, Gz, dpdx, rho_av, LES,
+                     isReadMovingWalls, omegaB, spzWidth, clim);
 }
 
 }  // extern "C"

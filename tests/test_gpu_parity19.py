@@ -14,75 +14,9 @@ from lbffs_amd import cases
 from conftest import rel_l2
 
 import pyoracle as po
+from parity_util import _vel, compare, parity, run_hip
 
 pytestmark = pytest.mark.gpu
-
-TOL = 1e-5
-
-
-def run_hip(case, nsteps, storage=lb.STORE_F32, store_every=0, block=0, chunks=1, tune=None, autotune=False):
-    with lb.Lattice(case.nx, case.ny, case.nz, storage=storage, sgs=case.sgs, omega0=case.omega0,
-                    block=block) as lat:
-        case.upload_to(lat)
-        if tune is not None:
-            lat.set_launch(block or 256, tune)
-        if autotune:
-            lat.autotune(3, **case.params)
-        left = nsteps
-        for c in range(chunks):
-            n = left // (chunks - c)
-            lat.step(n, store_every=store_every, **case.params)
-            left -= n
-        out = {k: lat.download(k) for k in ("f", "rho", "u", "v", "w", "tauSGS", "Fwx", "Fwy", "Fwz")}
-        out["reduce"] = lat.reduce()
-    return out
-
-
-def _vel(d):
-    return np.concatenate([np.asarray(d["u"], np.float64), np.asarray(d["v"], np.float64),
-                           np.asarray(d["w"], np.float64)])
-
-
-def _errs(a, b, keys):
-    e = {}
-    for k in keys:
-        e[k] = rel_l2(_vel(a), _vel(b)) if k == "vel" else rel_l2(a[k], b[k])
-    return e
-
-
-def compare(h, o, keys=("rho", "vel", "f"), tol=TOL, label=""):
-    """Plain relative-L2 comparison (velocity compared as one vector field)."""
-    errs = _errs(h, o, keys)
-    print(label, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
-    for k, e in errs.items():
-        assert e <= tol, f"{label} {k}: rel L2 {e:.3e} > {tol}"
-    return errs
-
-
-def parity(case, nsteps, keys=("rho", "vel", "f"), tau_tol=1e-4, label="", **kw):
-    """HIP FP32 against the oracle in FP32 and FP64 (same Jacobi read order).
-
-    floor = |oracle32 - oracle64| is the FP32 rounding level of the case.  Pass if
-      |hip32 - oracle32| <= max(1e-5, 2 floor)      (the north-star 1e-5 where FP32 allows it)
-      |hip32 - oracle64| <= 2 floor + 1e-6          (no less accurate than a literal FP32 run)
-    tauSGS (a derived, strongly nonlinear field) is held to tau_tol against oracle32.
-    """
-    h = run_hip(case, nsteps, **kw)
-    o32 = po.run19(case, nsteps, order=po.JACOBI)
-    o64 = po.run19(case, nsteps, order=po.JACOBI, precision="f64")
-    e32 = _errs(h, o32, keys)
-    e64 = _errs(h, o64, keys)
-    fl = _errs(o32, o64, keys)
-    print(label, "hip-o32", {k: f"{v:.2e}" for k, v in e32.items()}, "hip-o64",
-          {k: f"{v:.2e}" for k, v in e64.items()}, "floor", {k: f"{v:.2e}" for k, v in fl.items()}, flush=True)
-    for k in keys:
-        assert e32[k] <= max(TOL, 2 * fl[k]), f"{label} {k}: |hip-o32| {e32[k]:.3e} (floor {fl[k]:.3e})"
-        assert e64[k] <= 2 * fl[k] + 1e-6, f"{label} {k}: |hip-o64| {e64[k]:.3e} (floor {fl[k]:.3e})"
-    if case.sgs != lb.SGS_NONE:
-        et = rel_l2(h["tauSGS"], o32["tauSGS"])
-        print(label, f"tauSGS {et:.2e}", flush=True)
-        assert et <= tau_tol, f"{label} tauSGS {et:.3e}"
-    return h, o32
 
 
 @pytest.mark.parametrize("sgs", [lb.SGS_WALE, lb.SGS_SMAGORINSKY, lb.SGS_NONE])

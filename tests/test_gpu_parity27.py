@@ -1,0 +1,189 @@
+"""GPU parity of the D3Q27 HIP step (cumulant collision, interpolated bounce-back)
+against the oracle (CPU restatement of D3Q27_opencl/kernel.cl:2711-2826) and against
+the reference kernel itself (D3Q27 kernel.cl built for gfx950, run with its own host
+loop).
+
+Tolerances as for D3Q19: |hip32 - oracle32| <= max(1e-5, 2 floor) and
+|hip32 - oracle64| <= 2 floor + 1e-6 in relative L2, floor = the FP32 rounding level
+(largest |oracle32 - oracle64| among literal FP32 runs built without and with contraction,
+each also started one ulp away).
+"""
+import numpy as np
+import pytest
+
+import lbffs_amd as lb
+from lbffs_amd import cases
+from conftest import rel_l2
+
+import pyoracle as po
+from parity_util import _vel, compare, parity, run_hip
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("sgs", [lb.SGS_WALE, lb.SGS_SMAGORINSKY, lb.SGS_NONE])
+def test_cavity27_vs_oracle(sgs):
+    c = cases.cavity(16, sgs=sgs, q=27)
+    c.params["LES"] = 1.0
+    parity(c, 80, label=f"cavity16 q27 sgs={sgs}")
+
+
+def test_cavity27_2d_vs_oracle():
+    c = cases.cavity(24, nz=1, q=27)
+    parity(c, 120, label="cavity24x24x1 q27")
+
+
+def test_channel27_force_vs_oracle():
+    # body force through the half-step velocity shift of the cumulant collision
+    c = cases.channel(nx=8, ny=20, nz=4, tau=0.8, force=1e-5, q=27)
+    parity(c, 200, label="channel q27")
+
+
+@pytest.mark.parametrize("sdf_from_geometry", [False, True])
+def test_cylinder27_ibb_vs_oracle(sdf_from_geometry):
+    # FixedVelocity inlet with noise, outlet + sponge, and solid links: with distances
+    # from the geometry both interpolated branches (q <= 1/2 and q > 1/2) run
+    c = cases.cylinder_channel(nx=64, ny=32, nz=6, D=10, cx=20, cy=16, Re=60.0, spz_cells=6, q=27,
+                               sdf_from_geometry=sdf_from_geometry)
+    h, o = parity(c, 100, label=f"cylinder q27 sdf={sdf_from_geometry}")
+    fw_h = np.array([h["Fwx"].astype(np.float64).sum(), h["Fwy"].astype(np.float64).sum()])
+    fw_o = np.array([o["Fwx"].astype(np.float64).sum(), o["Fwy"].astype(np.float64).sum()])
+    assert rel_l2(fw_h, fw_o) <= 1e-4, (fw_h, fw_o)
+
+
+def test_mixed_faces27_vs_oracle():
+    # FixedDensity outlet, Symmetry (mirrored links) and WallFunction faces, FixedVelocity inlet
+    codes = {f: ("Cyclic", "Cyclic", "Cyclic") for f in cases.FACES}
+    codes["nxMin"] = ("FixedVelocity", "Cyclic", "Cyclic")
+    codes["nxMax"] = ("FixedDensity", "Cyclic", "Cyclic")
+    codes["nyMin"] = ("Cyclic", "WallFunction", "Cyclic")
+    codes["nyMax"] = ("Cyclic", "Symmetry", "Cyclic")
+    codes["nzMin"] = ("Cyclic", "Cyclic", "BounceBack")
+    codes["nzMax"] = ("Cyclic", "Cyclic", "Symmetry")
+    c = cases.make_case("mixed27", 32, 16, 10, 1.0, 1.0, 0.05, 1e-3, codes, {"nxMin": (1.0, 0.0, 0.0)},
+                        sgs=lb.SGS_WALE, q=27)
+    parity(c, 60, label="mixed faces q27")
+
+
+def test_moving_zmax_lid27_vs_oracle():
+    # FixedVelocity on nzMax: the reference's nzMax corner-diagonal weights (K27:1376-1379)
+    codes = {f: ("BounceBack", "BounceBack", "BounceBack") for f in cases.FACES}
+    codes["nzMax"] = ("BounceBack", "BounceBack", "FixedVelocity")
+    c = cases.make_case("zlid27", 12, 12, 12, 1.0, 1.0, 0.1, 1e-3, codes, {"nzMax": (1.0, 0.5, 0.0)},
+                        sgs=lb.SGS_NONE, q=27)
+    parity(c, 60, label="z lid q27")
+
+
+def test_store_modes_bitwise_identical27():
+    c = cases.cylinder_channel(nx=48, ny=24, nz=4, D=8, cx=16, cy=12, spz_cells=4, sgs=lb.SGS_SMAGORINSKY, q=27,
+                               sdf_from_geometry=True)
+    a = run_hip(c, 30, store_every=0)
+    b = run_hip(c, 30, store_every=1)
+    d = run_hip(c, 30, store_every=0, chunks=3)
+    for k in ("f", "rho", "u", "v", "w", "tauSGS", "Fwx"):
+        assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(a[k], d[k]), k
+
+
+@pytest.mark.parametrize("tune", [1, 2, 3])
+def test_launch_options_bitwise_identical27(tune):
+    c = cases.cavity(16, q=27)
+    a = run_hip(c, 20)
+    b = run_hip(c, 20, tune=tune, block=128)
+    assert np.array_equal(a["f"], b["f"])
+
+
+@pytest.mark.parametrize("storage", [lb.STORE_F16DEV, lb.STORE_BF16DEV])
+def test_16bit_storage_accuracy27(storage):
+    c = cases.cylinder_channel(nx=96, ny=48, nz=8, D=12, cx=30, cy=24, Re=60.0, spz_cells=8, sgs=lb.SGS_SMAGORINSKY,
+                               q=27)
+    h = run_hip(c, 300, storage=storage)
+    d = run_hip(c, 300, storage=lb.STORE_F64)
+    e = rel_l2(_vel(h), _vel(d))
+    print("16-bit storage q27", storage, f"velocity rel L2 vs f64 {e:.2e}", flush=True)
+    # bfloat16 keeps 8 significant bits of each deviation: 2e-2 is its level on D3Q27
+    assert e <= (1e-3 if storage == lb.STORE_F16DEV else 2e-2)
+
+
+def test_f64_vs_oracle_f64_27():
+    c = cases.cylinder_channel(nx=48, ny=24, nz=4, D=8, cx=16, cy=12, spz_cells=4, q=27, sdf_from_geometry=True)
+    h = run_hip(c, 60, storage=lb.STORE_F64)
+    o = po.run19(c, 60, order=po.JACOBI, precision="f64")
+    compare(h, o, keys=("rho", "vel", "f", "tauSGS"), tol=1e-11, label="f64 q27")
+
+
+def test_reference_kernel27_cavity_short():
+    """The reference D3Q27 kernel.cl on the GPU against the oracle and the HIP step."""
+    c = cases.cavity(24, q=27)
+    r = po.run_reference_gpu(c, 100)
+    o = po.run19(c, 100, order=po.JACOBI)
+    h = run_hip(c, 100)
+    compare(r, o, keys=("rho", "vel", "f"), tol=1e-3, label="refcl27 vs oracle")
+    compare(h, r, keys=("rho", "vel", "f"), tol=1e-3, label="hip27 vs refcl27")
+
+
+def _periodic_face_neisolid(c):
+    """Flag the fluid face cells of periodic faces that touch a solid cell (the host
+    helper leaves face cells out)."""
+    sol = c.arrays["solid"].reshape(c.nz, c.ny, c.nx).astype(bool)
+    near = np.zeros_like(sol)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                near |= np.roll(sol, (dz, dy, dx), axis=(0, 1, 2))
+    k = np.arange(c.nz)[:, None, None] * np.ones_like(sol)
+    zface = (k == 0) | (k == c.nz - 1)
+    add = (near & ~sol & zface).reshape(-1)
+    c.arrays["neiSolid"] = (c.arrays["neiSolid"].astype(bool) | add).astype(np.uint8)
+    return int(add.sum())
+
+
+def test_cylinder27_ibb_on_periodic_faces_vs_oracle():
+    # solid links of cells on the periodic z faces (sources found by wrapping)
+    c = cases.cylinder_channel(nx=48, ny=24, nz=4, D=8, cx=16, cy=12, Re=60.0, spz_cells=4, q=27,
+                               sdf_from_geometry=True)
+    assert _periodic_face_neisolid(c) > 0
+    parity(c, 60, label="cylinder q27 periodic-face IBB")
+
+
+def test_neisolid_on_wall_face_refused27():
+    # the reference reads solid[-1] for such a cell: the engine refuses the lattice
+    c = cases.cavity(12, q=27)
+    c.arrays["neiSolid"] = c.arrays["neiSolid"].copy()
+    c.arrays["neiSolid"][6 + 12 * 6] = 1  # (6, 6, 0): on the nzMin BounceBack wall
+    with pytest.raises(lb.LbffsError, match="wall face"):
+        run_hip(c, 1)
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_one_step_rounding_level(q):
+    """From a developed state, one HIP step against the FP64 oracle step from the same
+    FP32 state: per direction the HIP rounding error is no larger than a literal FP32
+    oracle step's (1.5x the rms of the build without or with contraction, whichever is
+    larger), and HIP and the FP32 oracle agree cell by cell to a few ulp of the largest
+    population."""
+    c = cases.cylinder_channel(nx=64, ny=32, nz=4, D=10, cx=20, cy=16, Re=60.0, spz_cells=6, q=q,
+                               sdf_from_geometry=(q == 27))
+    keys = ("f", "rho", "u", "v", "w", "tauSGS", "Fwx", "Fwy", "Fwz")
+    coll = lb.COLL_CUMULANT if q == 27 else lb.COLL_RR
+    with lb.Lattice(c.nx, c.ny, c.nz, q=q, sgs=c.sgs, collision=coll, omega0=c.omega0) as lat:
+        c.upload_to(lat)
+        lat.step(40, **c.params)
+        s40 = {k: lat.download(k) for k in keys}
+        lat.step(1, **c.params)
+        h = {k: lat.download(k) for k in keys}
+    st32, st64 = po.initial_state(c, np.float32), po.initial_state(c, np.float64)
+    for k in keys:
+        st32[k] = s40[k].astype(np.float32)
+        st64[k] = s40[k].astype(np.float64)
+    o32 = po.run19(c, 1, state=st32)
+    o32f = po.run19(c, 1, state=st32, fma=True)
+    o64 = po.run19(c, 1, precision="f64", state=st64)
+    eh = (h["f"].astype(np.float64) - o64["f"]).reshape(q, -1)
+    eo = (o32["f"].astype(np.float64) - o64["f"]).reshape(q, -1)
+    ef = (o32f["f"].astype(np.float64) - o64["f"]).reshape(q, -1)
+    rms_h = np.sqrt((eh ** 2).mean(1))
+    rms_o = np.maximum(np.sqrt((eo ** 2).mean(1)), np.sqrt((ef ** 2).mean(1)))
+    print("one step q", q, "rms ratio hip/o32 by q", np.array2string(rms_h / rms_o, precision=2), flush=True)
+    assert np.all(rms_h <= 1.5 * rms_o + 1e-12)
+    assert np.abs(h["f"].astype(np.float64) - o32["f"]).max() <= 64 * np.spacing(np.float32(0.3))

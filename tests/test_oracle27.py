@@ -50,12 +50,18 @@ def test_upwind_and_mirror_tables():
                                for s, n, b in ((si, nx, b1), (sj, ny, b2), (sk, nz, b3)))
                     exp = -1 if wall else (si % nx) + nx * ((sj % ny) + ny * (sk % nz))
                     assert L.orc27_upwind_B(q, i, j, k, nx, ny, nz, b1, b2, b3) == exp
+    # mirrorQx/y/z (K27:186-528) are tables: the axis component flipped, or the whole
+    # direction reflected when it has none (entries quoted from the tables below)
     for q in range(27):
         for ax in range(3):
             c = [CX[q], CY[q], CZ[q]]
-            c[ax] = -c[ax]
+            c = [-x for x in c] if c[ax] == 0 else [(-x if a == ax else x) for a, x in enumerate(c)]
             m = L.orc27_mirror(q, ax)
             assert (CX[m], CY[m], CZ[m]) == tuple(c)
+    quoted = {(3, 0): 4, (15, 0): 16, (19, 0): 21, (1, 1): 2, (11, 1): 12, (7, 1): 9, (7, 2): 8,
+              (19, 2): 25, (21, 2): 24, (26, 2): 20}
+    for (q, ax), m in quoted.items():
+        assert L.orc27_mirror(q, ax) == m
 
 
 @pytest.mark.parametrize("seed", range(5))
