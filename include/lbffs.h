@@ -158,6 +158,34 @@ int lbffs_autotune(lbffs_ctx *ctx, const lbffs_step_params *params, int trial_st
 /* Set the launch shape directly (work-group size multiple of 64 up to 256, LBFFS_TUNE_*). */
 int lbffs_set_launch(lbffs_ctx *ctx, int block, int tune);
 
+/* ---- immersed-boundary moving walls (k_Gwall / k_Gibm / k_Force, K19:3648-3807,
+ * K27:2828-2986) ----
+ * Markers are points in lattice coordinates (movingSTLcList, ns.cpp:248-254).  With more
+ * than one marker every step is followed, as in ns.cpp:544-600, by: G_m = U_wall - the
+ * trilinear interpolation of u over the marker's 8-cell box; G spread back over the box;
+ * the marker moved by the translation and turned by rot_omega about the axis; and
+ * f_q += rho w_q 3 (G . c_q) on the forced fluid cells.  The spread sums every marker's
+ * contribution (the reference's atom_add_float result is written back over the sum,
+ * K19:3760-3762).  GX/GY/GZ hold the spread G after a call, as the reference's GxIBM. */
+typedef struct {
+    float trans[3];      /* uMovingTrans, vMovingTrans, wMovingTrans per step (walls.hpp:606-611) */
+    float rot_center[3]; /* rotX, rotY, rotZ in lattice units (walls.hpp:615-620) */
+    float rot_axis[3];   /* rotAxisX, rotAxisY, rotAxisZ (walls.hpp:621-626) */
+    float rot_omega;     /* rotOmega in radians per step (walls.hpp:613-614) */
+    int reserved[6];
+} lbffs_ibm_motion;
+
+enum { LBFFS_IBM_POSITIONS = 0, LBFFS_IBM_FORCING = 1 };
+
+/* Set n markers (3n host reals, xyz interleaved; n <= 1 switches the moving walls off, as
+ * the reference host does when movingWalls.stl has one entry or none, ns.cpp:239-245, 544). */
+int lbffs_ibm_set_markers(lbffs_ctx *ctx, const void *xyz, int n);
+/* Motion of the markers (readMotions, walls.hpp:586-628, already in lattice units). */
+int lbffs_ibm_set_motion(lbffs_ctx *ctx, const lbffs_ibm_motion *motion);
+/* Copy out the marker positions (3n, xyz interleaved) or the marker forcing of the last
+ * step (3n: G_x of all markers, then G_y, then G_z; GxMovingWall...). */
+int lbffs_ibm_download(lbffs_ctx *ctx, int what, void *host, int n);
+
 /* Wait for queued device work. */
 int lbffs_synchronize(lbffs_ctx *ctx);
 

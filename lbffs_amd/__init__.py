@@ -51,6 +51,14 @@ class StepParams(C.Structure):
                 ("reserved", C.c_int * 7)]
 
 
+class IbmMotion(C.Structure):
+    _fields_ = [("trans", C.c_float * 3), ("rot_center", C.c_float * 3), ("rot_axis", C.c_float * 3),
+                ("rot_omega", C.c_float), ("reserved", C.c_int * 6)]
+
+
+IBM_POSITIONS, IBM_FORCING = 0, 1
+
+
 class CaseIn(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("Lx", C.c_float), ("uMax", C.c_float),
                 ("rho0", C.c_float), ("U0", C.c_float), ("nu", C.c_float), ("dpdx", C.c_float),
@@ -96,10 +104,14 @@ def lib():
         L.lbffs_autotune.argtypes = [vp, C.POINTER(StepParams), ip, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_float)]
         L.lbffs_set_launch.argtypes = [vp, ip, ip]
+        L.lbffs_ibm_set_markers.argtypes = [vp, vp, ip]
+        L.lbffs_ibm_set_motion.argtypes = [vp, C.POINTER(IbmMotion)]
+        L.lbffs_ibm_download.argtypes = [vp, ip, vp, ip]
         for n in ("lbffs_create", "lbffs_upload", "lbffs_download", "lbffs_step", "lbffs_reduce",
                   "lbffs_synchronize", "lbffs_last_step_ms", "lbffs_destroy", "lbffs_case_units",
                   "lbffs_case_faces", "lbffs_case_init_feq", "lbffs_case_neisolid", "lbffs_case_outlet_sponge",
-                  "lbffs_probe_copy", "lbffs_probe_bandwidth", "lbffs_autotune", "lbffs_set_launch"):
+                  "lbffs_probe_copy", "lbffs_probe_bandwidth", "lbffs_autotune", "lbffs_set_launch",
+                  "lbffs_ibm_set_markers", "lbffs_ibm_set_motion", "lbffs_ibm_download"):
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib
@@ -180,6 +192,25 @@ class Lattice:
 
     def set_launch(self, block=256, tune=0):
         _check(lib().lbffs_set_launch(self._h, int(block), int(tune)), "lbffs_set_launch")
+
+    def ibm_set_markers(self, xyz):
+        """Marker positions (n x 3 or 3n, lattice coordinates); n <= 1 switches IBM off."""
+        a = np.ascontiguousarray(np.asarray(xyz, dtype=self.real).reshape(-1))
+        _check(lib().lbffs_ibm_set_markers(self._h, _ptr(a), a.size // 3), "lbffs_ibm_set_markers")
+        self.n_markers = a.size // 3
+
+    def ibm_set_motion(self, trans=(0.0, 0.0, 0.0), center=(0.0, 0.0, 0.0), axis=(0.0, 0.0, 0.0), omega=0.0):
+        m = IbmMotion()
+        for a in range(3):
+            m.trans[a], m.rot_center[a], m.rot_axis[a] = trans[a], center[a], axis[a]
+        m.rot_omega = omega
+        _check(lib().lbffs_ibm_set_motion(self._h, C.byref(m)), "lbffs_ibm_set_motion")
+
+    def ibm_download(self, what=IBM_POSITIONS):
+        n = getattr(self, "n_markers", 0)
+        a = np.empty(3 * n, dtype=self.real)
+        _check(lib().lbffs_ibm_download(self._h, int(what), _ptr(a), n), "lbffs_ibm_download")
+        return a
 
     def reduce(self):
         out = (C.c_double * REDUCE_N)()

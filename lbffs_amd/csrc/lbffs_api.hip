@@ -12,6 +12,7 @@
 #include "../../include/lbffs.h"
 #include "d3q19_step.hpp"
 #include "d3q27_step.hpp"
+#include "ibm.hpp"
 
 using namespace lbffs;
 
@@ -71,6 +72,16 @@ struct lbffs_ctx {
     bool timed = false;
     int block = 256;  // work-group size of the step launch
     int tune = 0;     // TUNE_* launch options
+    // immersed-boundary moving walls (active with more than one marker, ns.cpp:544)
+    int n_markers = 0;
+    void *markers = nullptr;   // 3n host reals, xyz interleaved
+    void *Gm = nullptr;        // 3n host reals: G_x[n], G_y[n], G_z[n]
+    uint32_t *mark = nullptr;  // per-cell stamp of the last spread that touched it
+    int *list = nullptr;       // cells touched by the last spread (capacity 8n)
+    int *count = nullptr;
+    uint32_t stamp = 0;
+    double motion[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool g_dirty = true;       // G may hold values outside the last list
 };
 
 namespace {
@@ -389,6 +400,62 @@ void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream
         hipLaunchKernelGGL((step19_kernel<S, Real, SGS>), dim3(grid), dim3(block), 0, st, a);
 }
 
+template <typename Real>
+IbmMotion<Real> ibm_motion(const lbffs_ctx *c)
+{
+    IbmMotion<Real> m;
+    for (int a = 0; a < 3; a++) {
+        m.trans[a] = (Real)c->motion[a];
+        m.centre[a] = (Real)c->motion[3 + a];
+        m.axis[a] = (Real)c->motion[6 + a];
+    }
+    m.omega = (Real)c->motion[9];
+    return m;
+}
+
+bool ibm_active(const lbffs_ctx *c) { return c->n_markers > 1; }
+
+// the moving-wall kernels after a step (ns.cpp:544-600): marker forcing from the new
+// velocities, spread + marker motion, forcing of the new populations
+template <typename S, typename Real, int Q>
+int ibm_after_step(lbffs_ctx *c, bool last)
+{
+    const int n = c->n_markers, cap = 8 * n;
+    const IbmMotion<Real> m = ibm_motion<Real>(c);
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    c->stamp = (c->stamp == 0xffffffffu) ? 1u : c->stamp + 1u;
+    hipLaunchKernelGGL((ibm_gwall_kernel<Real>), dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n,
+                       (const Real *)c->markers, nx, ny, nz, (const Real *)c->u[c->mcur], (const Real *)c->v[c->mcur],
+                       (const Real *)c->w[c->mcur], m, (Real *)c->Gm, c->count);
+    hipLaunchKernelGGL((ibm_spread_move_kernel<Real>), dim3(grid_for(n, 256)), dim3(256), 0, c->stream, n,
+                       (Real *)c->markers, nx, ny, nz, (const Real *)c->Gm, m, (Real *)c->Gx, (Real *)c->Gy,
+                       (Real *)c->Gz, c->mark, c->stamp, c->list, c->count, cap);
+    hipLaunchKernelGGL((ibm_force_kernel<S, Real, Q>), dim3(grid_for(cap, 256)), dim3(256), 0, c->stream,
+                       (const int *)c->list, (const int *)c->count, cap, (uint32_t)c->N, (S *)c->f[c->cur], c->flag,
+                       (const Real *)c->rho[c->mcur], (Real *)c->Gx, (Real *)c->Gy, (Real *)c->Gz, last ? 0 : 1);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// G = 0 where the next spread must start from zero
+template <typename Real>
+int ibm_clear_g(lbffs_ctx *c)
+{
+    if (c->g_dirty) {
+        for (void *g : {c->Gx, c->Gy, c->Gz})
+            hipLaunchKernelGGL((fill_kernel<Real>), dim3(1024), dim3(256), 0, c->stream, (Real *)g, (size_t)c->N,
+                               Real(0));
+        c->g_dirty = false;
+    } else {
+        const int cap = 8 * c->n_markers;
+        hipLaunchKernelGGL((ibm_zero_listed_kernel<Real>), dim3(grid_for(cap, 256)), dim3(256), 0, c->stream,
+                           (const int *)c->list, (const int *)c->count, cap, (Real *)c->Gx, (Real *)c->Gy,
+                           (Real *)c->Gz);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 template <typename S, typename Real, int Q>
 int run_steps_q(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
 {
@@ -396,6 +463,10 @@ int run_steps_q(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
     set_args<S, Real, Q>(c, a, p);
     const int block = c->block;
     const unsigned grid = grid_for(c->N, block);
+    const bool ibm = ibm_active(c);
+    if (ibm) {
+        if (int e = ibm_clear_g<Real>(c)) return e;
+    }
     for (int s = 0; s < nsteps; s++) {
         const bool last = (s == nsteps - 1);
         a.fsrc = c->f[c->cur];
@@ -409,7 +480,9 @@ int run_steps_q(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
         a.v_n = (Real *)c->v[1 - c->mcur];
         a.w_n = (Real *)c->w[1 - c->mcur];
         a.aux_all = (p->store_every || last) ? 1 : 0;
-        a.mac_all = (a.aux_all || c->d.sgs == LBFFS_SGS_WALE) ? 1 : 0;
+        // WALE reads every neighbour's velocity; the moving-wall kernels read the
+        // velocities of the marker boxes and the density of the cells they force
+        a.mac_all = (a.aux_all || c->d.sgs == LBFFS_SGS_WALE || ibm) ? 1 : 0;
         switch (c->d.sgs) {
         case LBFFS_SGS_WALE: launch_step<S, Real, Q, 0>(a, grid, block, c->stream); break;
         case LBFFS_SGS_SMAGORINSKY: launch_step<S, Real, Q, 1>(a, grid, block, c->stream); break;
@@ -418,6 +491,9 @@ int run_steps_q(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
         HIP_TRY(hipGetLastError());
         c->cur ^= 1;
         c->mcur ^= 1;
+        if (ibm) {
+            if (int e = ibm_after_step<S, Real, Q>(c, last)) return e;
+        }
     }
     return 0;
 }
@@ -537,7 +613,8 @@ void free_all(lbffs_ctx *c)
 {
     void *ptrs[] = {c->f[0], c->f[1], c->boundary, c->b1, c->b2, c->b3, c->solid, c->neiSolid, c->sdf, c->u0, c->v0,
                     c->w0, c->omega, c->Fwx, c->Fwy, c->Fwz, c->tau, c->Gx, c->Gy, c->Gz, c->rho[0], c->rho[1],
-                    c->u[0], c->u[1], c->v[0], c->v[1], c->w[0], c->w[1], c->flag, c->d_err, c->d_part, c->d_red};
+                    c->u[0], c->u[1], c->v[0], c->v[1], c->w[0], c->w[1], c->flag, c->d_err, c->d_part, c->d_red,
+                    c->markers, c->Gm, c->mark, c->list, c->count};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -642,6 +719,7 @@ int lbffs_upload(lbffs_ctx *c, int field, const void *host, size_t bytes)
     }
     HIP_TRY(hipMemcpy(field_ptr(c, field), host, bytes, hipMemcpyHostToDevice));
     if (field_affects_flags(field)) c->flags_dirty = true;
+    if (field == LBFFS_FIELD_GX || field == LBFFS_FIELD_GY || field == LBFFS_FIELD_GZ) c->g_dirty = true;
     return 0;
 }
 
@@ -685,11 +763,21 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
     }
     void *keep[11] = {c->rho[c->mcur], c->u[c->mcur], c->v[c->mcur], c->w[c->mcur], c->tau,
                       c->Fwx, c->Fwy, c->Fwz, c->Gx, c->Gy, c->Gz};
+    const size_t mbytes = (size_t)3 * (size_t)c->n_markers * rs;
+    void *saved_m = nullptr;
+    if (c->n_markers > 0) {
+        if (int e = dalloc(&saved_m, mbytes)) {
+            (void)hipFree(saved_f);
+            (void)hipFree(saved_r);
+            return e;
+        }
+    }
     const int cur0 = c->cur, mcur0 = c->mcur, block0 = c->block, tune0 = c->tune;
     int rc = 0;
     hipError_t he = hipMemcpyAsync(saved_f, c->f[cur0], fbytes, hipMemcpyDeviceToDevice, c->stream);
     for (int k = 0; k < 11 && he == hipSuccess; k++)
         he = hipMemcpyAsync((char *)saved_r + k * n * rs, keep[k], n * rs, hipMemcpyDeviceToDevice, c->stream);
+    if (saved_m && he == hipSuccess) he = hipMemcpyAsync(saved_m, c->markers, mbytes, hipMemcpyDeviceToDevice, c->stream);
     const int blocks[3] = {64, 128, 256};
     const int tunes[4] = {0, TUNE_NT_STORE, TUNE_NO_XCD, TUNE_NT_STORE | TUNE_NO_XCD};
     float bms = 1e30f;
@@ -722,9 +810,12 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
     if (he == hipSuccess) he = hipMemcpyAsync(c->f[cur0], saved_f, fbytes, hipMemcpyDeviceToDevice, c->stream);
     for (int k = 0; k < 11 && he == hipSuccess; k++)
         he = hipMemcpyAsync(keep[k], (char *)saved_r + k * n * rs, n * rs, hipMemcpyDeviceToDevice, c->stream);
+    if (saved_m && he == hipSuccess) he = hipMemcpyAsync(c->markers, saved_m, mbytes, hipMemcpyDeviceToDevice, c->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
     (void)hipFree(saved_f);
     (void)hipFree(saved_r);
+    if (saved_m) (void)hipFree(saved_m);
+    c->g_dirty = true;  // the restored G are the pre-tuning ones, not those of the last list
     if (rc) {
         c->block = block0;
         c->tune = tune0;
@@ -751,6 +842,56 @@ int lbffs_set_launch(lbffs_ctx *c, int block, int tune)
     if (tune & ~(TUNE_NT_STORE | TUNE_NO_XCD)) return fail(LBFFS_E_INVALID, "unknown launch option");
     c->block = block;
     c->tune = tune;
+    return 0;
+}
+
+int lbffs_ibm_set_markers(lbffs_ctx *c, const void *xyz, int n)
+{
+    if (!c || (n > 0 && !xyz)) return fail(LBFFS_E_INVALID, "null argument");
+    if (n < 0 || n > (1 << 27)) return fail(LBFFS_E_INVALID, "marker count out of range");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    void *ptrs[] = {c->markers, c->Gm, c->mark, c->list, c->count};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    c->markers = c->Gm = nullptr;
+    c->mark = nullptr;
+    c->list = c->count = nullptr;
+    c->n_markers = 0;
+    if (n == 0) return 0;
+    const size_t rs = c->real_size, m3 = (size_t)3 * n * rs;
+    int e = dalloc(&c->markers, m3);
+    if (!e) e = dalloc(&c->Gm, m3);
+    if (!e) e = dalloc((void **)&c->mark, (size_t)c->N * sizeof(uint32_t));
+    if (!e) e = dalloc((void **)&c->list, (size_t)8 * n * sizeof(int));
+    if (!e) e = dalloc((void **)&c->count, sizeof(int));
+    if (e) return e;
+    HIP_TRY(hipMemcpy(c->markers, xyz, m3, hipMemcpyHostToDevice));
+    c->n_markers = n;
+    c->stamp = 0;
+    c->g_dirty = true;
+    return 0;
+}
+
+int lbffs_ibm_set_motion(lbffs_ctx *c, const lbffs_ibm_motion *m)
+{
+    if (!c || !m) return fail(LBFFS_E_INVALID, "null argument");
+    for (int a = 0; a < 3; a++) {
+        c->motion[a] = m->trans[a];
+        c->motion[3 + a] = m->rot_center[a];
+        c->motion[6 + a] = m->rot_axis[a];
+    }
+    c->motion[9] = m->rot_omega;
+    return 0;
+}
+
+int lbffs_ibm_download(lbffs_ctx *c, int what, void *host, int n)
+{
+    if (!c || !host) return fail(LBFFS_E_INVALID, "null argument");
+    if (n != c->n_markers || n == 0) return fail(LBFFS_E_INVALID, "marker count differs from the markers set");
+    if (what != LBFFS_IBM_POSITIONS && what != LBFFS_IBM_FORCING) return fail(LBFFS_E_INVALID, "what");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(host, what == LBFFS_IBM_POSITIONS ? c->markers : c->Gm, (size_t)3 * n * c->real_size,
+                      hipMemcpyDeviceToHost));
     return 0;
 }
 

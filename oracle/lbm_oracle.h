@@ -75,6 +75,25 @@ void orc27_collide_f32(const float *ft, float rho, float u, float v, float w, fl
 void orc27_collide_f64(const double *ft, double rho, double u, double v, double w, double tau, double tauSGS,
                        double omegaB, double clim, double *out);
 
+/* immersed boundary (moving walls); m[10] = trans[3], rot centre[3], rot axis[3], omega */
+long orc_ibm_icbox(long ic, int b, int nx, int ny, int nz);
+void orc_ibm_urot_f32(const float *m, float x, float y, float z, float *ur, float *vr, float *wr);
+void orc_ibm_xrot_f32(const float *m, float x, float y, float z, float *X, float *Y, float *Z);
+void orc_ibm_gwall_f32(const float *m, int n, const float *xyz, int nx, int ny, int nz, const float *u,
+                       const float *v, const float *w, float *Gx, float *Gy, float *Gz);
+void orc_ibm_spread_move_f32(const float *m, int n, float *xyz, int nx, int ny, int nz, const float *Gmx,
+                             const float *Gmy, const float *Gmz, float *Gx, float *Gy, float *Gz);
+void orc_ibm_force_f32(int Q, long N, float *f, const unsigned char *solid, const float *rho, const float *Gx,
+                       const float *Gy, const float *Gz);
+void orc_ibm_urot_f64(const double *m, double x, double y, double z, double *ur, double *vr, double *wr);
+void orc_ibm_xrot_f64(const double *m, double x, double y, double z, double *X, double *Y, double *Z);
+void orc_ibm_gwall_f64(const double *m, int n, const double *xyz, int nx, int ny, int nz, const double *u,
+                       const double *v, const double *w, double *Gx, double *Gy, double *Gz);
+void orc_ibm_spread_move_f64(const double *m, int n, double *xyz, int nx, int ny, int nz, const double *Gmx,
+                             const double *Gmy, const double *Gmz, double *Gx, double *Gy, double *Gz);
+void orc_ibm_force_f64(int Q, long N, double *f, const unsigned char *solid, const double *rho, const double *Gx,
+                       const double *Gy, const double *Gz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,12 +72,28 @@ def _bind(L):
     L.orc_upwind_B.restype = C.c_long
     L.orc_reflect_or_mirror.argtypes = [C.c_int] * 4
     L.orc_corner.argtypes = [C.c_int] * 7
+    L.orc_ibm_icbox.argtypes = [C.c_long, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.orc_ibm_icbox.restype = C.c_long
+    for p, fl in (("f32", C.c_float), ("f64", C.c_double)):
+        getattr(L, f"orc_ibm_urot_{p}").argtypes = [vp, fl, fl, fl, vp, vp, vp]
+        getattr(L, f"orc_ibm_xrot_{p}").argtypes = [vp, fl, fl, fl, vp, vp, vp]
+        getattr(L, f"orc_ibm_gwall_{p}").argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int] + [vp] * 6
+        getattr(L, f"orc_ibm_spread_move_{p}").argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int] + [vp] * 6
+        getattr(L, f"orc_ibm_force_{p}").argtypes = [C.c_int, C.c_long] + [vp] * 6
     return L
 
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
+
+# direction sets (K19:3558-3560, K27:2742-2747)
+ORC19_C = ([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0],
+           [0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1],
+           [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1])
+ORC27_C = (ORC19_C[0] + [1, -1, -1, 1, 1, -1, 1, -1],
+           ORC19_C[1] + [1, -1, 1, -1, -1, 1, 1, -1],
+           ORC19_C[2] + [1, -1, 1, -1, 1, -1, -1, 1])
 
 MACRO = ["Fwx", "Fwy", "Fwz", "tauSGS", "rho", "u", "v", "w", "GxIBM", "GyIBM", "GzIBM"]
 
@@ -91,10 +107,14 @@ def initial_state(case, real=np.float32):
     return st
 
 
-def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=None, sgs=None, fma=False):
+def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=None, sgs=None, fma=False,
+          ibm=None):
     """Run nsteps of the restated reference step (D3Q19, or D3Q27 when the case has
     q == 27).  Returns the state dict (f = current populations after the swaps, plus the
-    macro arrays)."""
+    macro arrays).  ibm = {"xyz": marker positions (3n), "motion": 10 motion parameters}
+    adds the moving-wall kernels after each step as the reference host does when more
+    than one marker exists (ns.cpp:544-600); the state then carries "ibm_xyz" and
+    "ibm_G" (3n, the marker forcing of the last step)."""
     real = np.float64 if precision == "f64" else np.float32
     a = case.arrays
     st = initial_state(case, real) if state is None else {k: v.copy() for k, v in state.items()}
@@ -109,6 +129,12 @@ def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=
     f = st["f"]
     ft = f.copy()  # initialization.hpp: fTmp = f
     fn = getattr(orc(fma), f"orc{getattr(case, 'q', 19)}_step_{precision}")
+    if ibm is not None:
+        xyz = np.ascontiguousarray(st.get("ibm_xyz", ibm["xyz"]), real).copy()
+        mot = np.ascontiguousarray(ibm["motion"], real)
+        nm = xyz.size // 3
+        Gm = np.zeros((3, nm), real)
+        L = orc(fma)
     for _ in range(nsteps):
         rc = fn(C.byref(P), _p(f), _p(ft), _p(ints["boundary"]), _p(ints["boundary1"]), _p(ints["boundary2"]),
                 _p(ints["boundary3"]), _p(sides["sdf"]), _p(u8["solid"]), _p(u8["neiSolid"]), _p(sides["u0"]),
@@ -118,7 +144,18 @@ def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=
         if rc != 0:
             raise RuntimeError("oracle step failed")
         f, ft = ft, f
+        if ibm is not None and nm > 1:
+            getattr(L, f"orc_ibm_gwall_{precision}")(_p(mot), nm, _p(xyz), case.nx, case.ny, case.nz, _p(st["u"]),
+                                                     _p(st["v"]), _p(st["w"]), _p(Gm[0]), _p(Gm[1]), _p(Gm[2]))
+            getattr(L, f"orc_ibm_spread_move_{precision}")(_p(mot), nm, _p(xyz), case.nx, case.ny, case.nz,
+                                                           _p(Gm[0]), _p(Gm[1]), _p(Gm[2]), _p(st["GxIBM"]),
+                                                           _p(st["GyIBM"]), _p(st["GzIBM"]))
+            getattr(L, f"orc_ibm_force_{precision}")(case.q, case.N, _p(f), _p(u8["solid"]), _p(st["rho"]),
+                                                     _p(st["GxIBM"]), _p(st["GyIBM"]), _p(st["GzIBM"]))
     st["f"] = f
+    if ibm is not None:
+        st["ibm_xyz"] = xyz
+        st["ibm_G"] = Gm.reshape(-1).copy()
     return st
 
 
@@ -160,6 +197,8 @@ def refrun():
         L.refrun_step19.restype = C.c_int
         L.refrun_step27.argtypes = L.refrun_step19.argtypes + [C.c_float]
         L.refrun_step27.restype = C.c_int
+        L.refrun_ibm.argtypes = [C.c_char_p] + [C.c_int] * 5 + [vp] * 10
+        L.refrun_ibm.restype = C.c_int
         L.refrun_error.restype = C.c_char_p
         _ref = L
     return _ref
@@ -201,6 +240,23 @@ def run_reference_gpu(case, nsteps, block=None, rho_av=1.0, LES=None, state=None
         raise RuntimeError("reference run failed: " + refrun().refrun_error().decode())
     st["f"] = f
     return st
+
+
+def run_reference_ibm(case, xyz, motion, u, v, w, rho, Gin, f):
+    """One pass of the reference's k_Gwall, k_Gibm (marker motion) and k_Force on the GPU.
+    Returns (moved xyz, marker forcing Gm (3n), f after k_Force with the forcing Gin)."""
+    co = REF_CO27 if case.q == 27 else REF_CO19
+    xyz = np.ascontiguousarray(xyz, np.float32).copy()
+    n = xyz.size // 3
+    Gm = np.zeros(3 * n, np.float32)
+    f = np.ascontiguousarray(f, np.float32).copy()
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (u, v, w, rho, motion, Gin)]
+    sol = np.ascontiguousarray(case.arrays["solid"], np.float32)
+    rc = refrun().refrun_ibm(co.encode(), case.q, case.nx, case.ny, case.nz, n, _p(xyz), _p(arrs[0]), _p(arrs[1]),
+                             _p(arrs[2]), _p(arrs[3]), _p(arrs[4]), _p(Gm), _p(arrs[5]), _p(f), _p(sol))
+    if rc != 0:
+        raise RuntimeError("reference IBM run failed: " + refrun().refrun_error().decode())
+    return xyz, Gm, f
 
 
 def race_free_state(case, warm, iters=16):

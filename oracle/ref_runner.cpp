@@ -155,4 +155,80 @@ int refrun_step27(const char *co_path, int nx, int ny, int nz, int nsteps, int b
                      isReadMovingWalls, omegaB, spzWidth, clim);
 }
 
+// One pass of the reference's moving-wall kernels (K19:3648-3807 / K27:2828-2986) on given
+// inputs: k_Gwall (marker forcing -> Gm, 3n as GxMovingWall, GyMovingWall, GzMovingWall),
+// k_Gibm (its spread goes to a scratch lattice that is not returned -- it writes the value
+// before the atomic add back over the sum, K19:3760-3762 -- and its marker motion updates
+// xyz), and k_Force on the caller's G (Gin, 3N) adding to f (Q*N).  k_Force declares its
+// solid argument float* (K19:3784): it is given the flags as floats.
+int refrun_ibm(const char *co_path, int Q, int nx, int ny, int nz, int n, float *xyz, const float *u, const float *v,
+               const float *w, const float *rho, const float *motion, float *Gm, const float *Gin, float *f,
+               const float *solid_f)
+{
+    const long N = (long)nx * ny * nz;
+    if (n < 1 || N <= 0 || (long)Q * N >= (1L << 31)) return err("sizes");
+    auto divisor = [](long m) {
+        for (int b : {256, 128, 64, 32, 16, 8, 4, 2, 1})
+            if (m % b == 0) return b;
+        return 1;
+    };
+    hipModule_t mod;
+    H(hipModuleLoad(&mod, co_path));
+    hipFunction_t fg, fi, ff;
+    if (hipModuleGetFunction(&fg, mod, "k_Gwall") != hipSuccess || hipModuleGetFunction(&fi, mod, "k_Gibm") != hipSuccess ||
+        hipModuleGetFunction(&ff, mod, "k_Force") != hipSuccess) {
+        (void)hipModuleUnload(mod);
+        return err("k_Gwall/k_Gibm/k_Force not found");
+    }
+    const size_t nf = (size_t)N * sizeof(float), m3 = (size_t)3 * n * sizeof(float);
+    Buf RHO, U, V, W, XYZ, GMX, GMY, GMZ, SGX, SGY, SGZ, GX, GY, GZ, F, SOL;
+    struct {
+        Buf *b;
+        const void *src;
+        size_t bytes;
+    } up[] = {{&RHO, rho, nf}, {&U, u, nf}, {&V, v, nf}, {&W, w, nf}, {&XYZ, xyz, m3},
+              {&GMX, nullptr, m3 / 3}, {&GMY, nullptr, m3 / 3}, {&GMZ, nullptr, m3 / 3},
+              {&SGX, nullptr, nf}, {&SGY, nullptr, nf}, {&SGZ, nullptr, nf},
+              {&GX, Gin, nf}, {&GY, Gin + N, nf}, {&GZ, Gin + 2 * N, nf}, {&F, f, (size_t)Q * nf}, {&SOL, solid_f, nf}};
+    int rc = 0;
+    for (auto &x : up) {
+        if (hipMalloc(&x.b->p, x.bytes) != hipSuccess) {
+            rc = err("hipMalloc");
+            break;
+        }
+        hipError_t e = x.src ? hipMemcpy(x.b->p, x.src, x.bytes, hipMemcpyHostToDevice) : hipMemset(x.b->p, 0, x.bytes);
+        if (e != hipSuccess) {
+            rc = err("upload");
+            break;
+        }
+    }
+    if (rc == 0) {
+        int nM = n;
+        float ut = motion[0], vt = motion[1], wt = motion[2], rx = motion[3], ry = motion[4], rz = motion[5],
+              ax = motion[6], ay = motion[7], az = motion[8], om = motion[9];
+        unsigned elements = (unsigned)N;
+        void *pg[] = {&RHO.p, &U.p, &V.p, &W.p, &XYZ.p, &GMX.p, &GMY.p, &GMZ.p, &nM, &nx, &ny, &nz,
+                      &ut, &vt, &wt, &rx, &ry, &rz, &ax, &ay, &az, &om};
+        void *pi[] = {&RHO.p, &SGX.p, &SGY.p, &SGZ.p, &XYZ.p, &GMX.p, &GMY.p, &GMZ.p, &nM, &nx, &ny, &nz,
+                      &ut, &vt, &wt, &rx, &ry, &rz, &ax, &ay, &az, &om};
+        void *pf[] = {&F.p, &SOL.p, &RHO.p, &GX.p, &GY.p, &GZ.p, &elements};
+        const int bm = divisor(n), bn = divisor(N);
+        hipError_t e = hipModuleLaunchKernel(fg, (unsigned)(n / bm), 1, 1, bm, 1, 1, 0, nullptr, pg, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipModuleLaunchKernel(fi, (unsigned)(n / bm), 1, 1, bm, 1, 1, 0, nullptr, pi, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipModuleLaunchKernel(ff, (unsigned)(N / bn), 1, 1, bn, 1, 1, 0, nullptr, pf, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = err(std::string("launch: ") + hipGetErrorString(e));
+        if (rc == 0 && (hipMemcpy(xyz, XYZ.p, m3, hipMemcpyDeviceToHost) != hipSuccess ||
+                        hipMemcpy(Gm, GMX.p, m3 / 3, hipMemcpyDeviceToHost) != hipSuccess ||
+                        hipMemcpy(Gm + n, GMY.p, m3 / 3, hipMemcpyDeviceToHost) != hipSuccess ||
+                        hipMemcpy(Gm + 2 * n, GMZ.p, m3 / 3, hipMemcpyDeviceToHost) != hipSuccess ||
+                        hipMemcpy(f, F.p, (size_t)Q * nf, hipMemcpyDeviceToHost) != hipSuccess))
+            rc = err("D2H");
+    }
+    (void)hipModuleUnload(mod);
+    return rc;
+}
+
 }  // extern "C"

@@ -10,11 +10,19 @@ import pyoracle as po
 TOL = 1e-5
 
 
-def run_hip(case, nsteps, storage=lb.STORE_F32, store_every=0, block=0, chunks=1, tune=None, autotune=False):
+def _ibm_motion_kw(m):
+    return dict(trans=tuple(m[0:3]), center=tuple(m[3:6]), axis=tuple(m[6:9]), omega=float(m[9]))
+
+
+def run_hip(case, nsteps, storage=lb.STORE_F32, store_every=0, block=0, chunks=1, tune=None, autotune=False,
+            ibm=None):
     coll = lb.COLL_CUMULANT if case.q == 27 else lb.COLL_RR
     with lb.Lattice(case.nx, case.ny, case.nz, q=case.q, storage=storage, sgs=case.sgs, collision=coll,
                     omega0=case.omega0, block=block) as lat:
         case.upload_to(lat)
+        if ibm is not None:
+            lat.ibm_set_markers(ibm["xyz"])
+            lat.ibm_set_motion(**_ibm_motion_kw(ibm["motion"]))
         if tune is not None:
             lat.set_launch(block or 256, tune)
         if autotune:
@@ -26,6 +34,11 @@ def run_hip(case, nsteps, storage=lb.STORE_F32, store_every=0, block=0, chunks=1
             left -= n
         out = {k: lat.download(k) for k in ("f", "rho", "u", "v", "w", "tauSGS", "Fwx", "Fwy", "Fwz")}
         out["reduce"] = lat.reduce()
+        if ibm is not None:
+            out["ibm_xyz"] = lat.ibm_download(lb.IBM_POSITIONS)
+            out["ibm_G"] = lat.ibm_download(lb.IBM_FORCING)
+            for g in ("GxIBM", "GyIBM", "GzIBM"):
+                out[g] = lat.download(g)
     return out
 
 
@@ -72,9 +85,10 @@ def parity(case, nsteps, keys=("rho", "vel", "f"), tau_tol=1e-4, label="", **kw)
       |hip32 - oracle64| <= 2 floor + 1e-6          (no less accurate than a literal FP32 run)
     tauSGS (a derived, strongly nonlinear field) is held to tau_tol against oracle32.
     """
+    ibm = kw.get("ibm")
     h = run_hip(case, nsteps, **kw)
-    o32 = po.run19(case, nsteps, order=po.JACOBI)
-    o64 = po.run19(case, nsteps, order=po.JACOBI, precision="f64")
+    o32 = po.run19(case, nsteps, order=po.JACOBI, ibm=ibm)
+    o64 = po.run19(case, nsteps, order=po.JACOBI, precision="f64", ibm=ibm)
     e32 = _errs(h, o32, keys)
     e64 = _errs(h, o64, keys)
     fl = _errs(o32, o64, keys)
@@ -83,7 +97,7 @@ def parity(case, nsteps, keys=("rho", "vel", "f"), tau_tol=1e-4, label="", **kw)
             if not fma and seed is None:
                 continue
             st = None if seed is None else _ulp_perturbed_state(case, seed)
-            op = po.run19(case, nsteps, order=po.JACOBI, state=st, fma=fma)
+            op = po.run19(case, nsteps, order=po.JACOBI, state=st, fma=fma, ibm=ibm)
             fp = _errs(op, o64, keys)
             fl = {k: max(fl[k], fp[k]) for k in keys}
     print(label, "hip-o32", {k: f"{v:.2e}" for k, v in e32.items()}, "hip-o64",
@@ -95,4 +109,4 @@ def parity(case, nsteps, keys=("rho", "vel", "f"), tau_tol=1e-4, label="", **kw)
         et = rel_l2(h["tauSGS"], o32["tauSGS"])
         print(label, f"tauSGS {et:.2e}", flush=True)
         assert et <= tau_tol, f"{label} tauSGS {et:.3e}"
-    return h, o32
+    return h, o32, o64

@@ -41,6 +41,13 @@ def test_strerror_and_invalid_arguments():
     assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -1
     d = lb.Desc(abi_version=lb.ABI_VERSION, q=19, nx=4, ny=4, nz=4, storage=9)
     assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    # lattice/collision pairs of the reference kernels only (K19:3637 RR, K27:2824 cumulant)
+    for q, coll in ((27, lb.COLL_RR), (19, lb.COLL_CUMULANT), (21, lb.COLL_RR)):
+        d = lb.Desc(abi_version=lb.ABI_VERSION, q=q, nx=4, ny=4, nz=4, collision=coll)
+        assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -4
+    assert L.lbffs_ibm_set_markers(None, None, 0) == -1
+    assert L.lbffs_ibm_set_motion(None, None) == -1
+    assert L.lbffs_ibm_download(None, 0, None, 0) == -1
     assert L.lbffs_upload(None, 0, None, 0) == -1
     assert L.lbffs_step(None, 1, None) == -1
     assert L.lbffs_destroy(None) == 0

@@ -39,7 +39,7 @@ def test_poiseuille_force_vs_oracle():
 def test_cylinder_channel_vs_oracle():
     # inlet FixedVelocity with noise, Geier outlet + sponge, solid-link bounce-back and force
     c = cases.cylinder_channel(nx=96, ny=48, nz=8, D=12, cx=30, cy=24, Re=60.0, spz_cells=8)
-    h, o = parity(c, 150, label="cylinder")
+    h, o, _ = parity(c, 150, label="cylinder")
     fw_h = np.array([h["Fwx"].astype(np.float64).sum(), h["Fwy"].astype(np.float64).sum()])
     fw_o = np.array([o["Fwx"].astype(np.float64).sum(), o["Fwy"].astype(np.float64).sum()])
     assert rel_l2(fw_h, fw_o) <= 1e-4, (fw_h, fw_o)

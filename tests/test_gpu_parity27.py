@@ -45,7 +45,7 @@ def test_cylinder27_ibb_vs_oracle(sdf_from_geometry):
     # from the geometry both interpolated branches (q <= 1/2 and q > 1/2) run
     c = cases.cylinder_channel(nx=64, ny=32, nz=6, D=10, cx=20, cy=16, Re=60.0, spz_cells=6, q=27,
                                sdf_from_geometry=sdf_from_geometry)
-    h, o = parity(c, 100, label=f"cylinder q27 sdf={sdf_from_geometry}")
+    h, o, _ = parity(c, 100, label=f"cylinder q27 sdf={sdf_from_geometry}")
     fw_h = np.array([h["Fwx"].astype(np.float64).sum(), h["Fwy"].astype(np.float64).sum()])
     fw_o = np.array([o["Fwx"].astype(np.float64).sum(), o["Fwy"].astype(np.float64).sum()])
     assert rel_l2(fw_h, fw_o) <= 1e-4, (fw_h, fw_o)
