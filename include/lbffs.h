@@ -186,6 +186,26 @@ int lbffs_ibm_set_motion(lbffs_ctx *ctx, const lbffs_ibm_motion *motion);
  * step (3n: G_x of all markers, then G_y, then G_z; GxMovingWall...). */
 int lbffs_ibm_download(lbffs_ctx *ctx, int what, void *host, int n);
 
+/* ---- batches: many independent lattices advanced by one launch per step ----
+ * No counterpart in the reference, which runs one lattice per process (ns.cpp).  The
+ * lattices share q, storage, sgs and collision; sizes, omega0 and step scalars are their
+ * own.  A packed table of 256-cell blocks over all lattices drives one kernel per step;
+ * the work-groups walk it with a grid-stride loop, so a grid smaller than the table gives
+ * persistent work-groups.  Each lattice is an ordinary context (lbffs_batch_context) for
+ * uploads, downloads and reductions; the batch owns and destroys them, and they share the
+ * batch's stream.  Moving-wall markers are not supported inside a batch. */
+typedef struct lbffs_batch lbffs_batch;
+int lbffs_batch_create(const lbffs_desc *descs, int n, lbffs_batch **out);
+int lbffs_batch_context(lbffs_batch *batch, int i, lbffs_ctx **ctx);
+/* grid: work-groups per launch (0 = one per 256-cell block); graph_chunk: steps captured
+ * into one hipGraph and replayed (0 = plain launches; the last step of a call is always a
+ * plain launch) */
+int lbffs_batch_set_launch(lbffs_batch *batch, int grid, int graph_chunk);
+/* nsteps steps of every lattice; params holds one entry per lattice */
+int lbffs_batch_step(lbffs_batch *batch, int nsteps, const lbffs_step_params *params);
+int lbffs_batch_last_step_ms(lbffs_batch *batch, float *ms);
+int lbffs_batch_destroy(lbffs_batch *batch);
+
 /* Wait for queued device work. */
 int lbffs_synchronize(lbffs_ctx *ctx);
 

@@ -107,11 +107,19 @@ def lib():
         L.lbffs_ibm_set_markers.argtypes = [vp, vp, ip]
         L.lbffs_ibm_set_motion.argtypes = [vp, C.POINTER(IbmMotion)]
         L.lbffs_ibm_download.argtypes = [vp, ip, vp, ip]
+        L.lbffs_batch_create.argtypes = [C.POINTER(Desc), ip, C.POINTER(vp)]
+        L.lbffs_batch_context.argtypes = [vp, ip, C.POINTER(vp)]
+        L.lbffs_batch_set_launch.argtypes = [vp, ip, ip]
+        L.lbffs_batch_step.argtypes = [vp, ip, C.POINTER(StepParams)]
+        L.lbffs_batch_last_step_ms.argtypes = [vp, C.POINTER(C.c_float)]
+        L.lbffs_batch_destroy.argtypes = [vp]
         for n in ("lbffs_create", "lbffs_upload", "lbffs_download", "lbffs_step", "lbffs_reduce",
                   "lbffs_synchronize", "lbffs_last_step_ms", "lbffs_destroy", "lbffs_case_units",
                   "lbffs_case_faces", "lbffs_case_init_feq", "lbffs_case_neisolid", "lbffs_case_outlet_sponge",
                   "lbffs_probe_copy", "lbffs_probe_bandwidth", "lbffs_autotune", "lbffs_set_launch",
-                  "lbffs_ibm_set_markers", "lbffs_ibm_set_motion", "lbffs_ibm_download"):
+                  "lbffs_ibm_set_markers", "lbffs_ibm_set_motion", "lbffs_ibm_download", "lbffs_batch_create",
+                  "lbffs_batch_context", "lbffs_batch_set_launch", "lbffs_batch_step", "lbffs_batch_last_step_ms",
+                  "lbffs_batch_destroy"):
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib
@@ -160,6 +168,18 @@ class Lattice:
         h = C.c_void_p()
         _check(lib().lbffs_create(C.byref(d), C.byref(h)), "lbffs_create")
         self._h = h
+
+    @classmethod
+    def _view(cls, handle, nx, ny, nz, q, storage):
+        """A Lattice on a context owned by a Batch (closing it does not destroy it)."""
+        self = cls.__new__(cls)
+        self.nx, self.ny, self.nz, self.q = nx, ny, nz, q
+        self.N = nx * ny * nz
+        self.storage = storage
+        self.real = np.float64 if storage == STORE_F64 else np.float32
+        self._h = handle
+        self._owner = False
+        return self
 
     def _dtype(self, name):
         if name in INT_FIELDS:
@@ -227,7 +247,65 @@ class Lattice:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().lbffs_destroy(self._h)
+            if getattr(self, "_owner", True):
+                lib().lbffs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Batch:
+    """Many independent lattices advanced by one launch per step (lbffs_batch_*).
+
+    lattices: sequence of (nx, ny, nz, omega0); all share q, storage, sgs, collision.
+    ``batch.lattice[i]`` is a Lattice view for uploads/downloads of lattice i."""
+
+    def __init__(self, lattices, q=19, storage=STORE_F32, sgs=SGS_WALE, collision=COLL_RR):
+        n = len(lattices)
+        descs = (Desc * n)()
+        for i, (nx, ny, nz, om) in enumerate(lattices):
+            descs[i] = Desc(abi_version=ABI_VERSION, q=q, nx=nx, ny=ny, nz=nz, storage=storage, sgs=sgs,
+                            collision=collision, omega0=om, block=0)
+        h = C.c_void_p()
+        _check(lib().lbffs_batch_create(descs, n, C.byref(h)), "lbffs_batch_create")
+        self._h = h
+        self.lattice = []
+        for i, (nx, ny, nz, _) in enumerate(lattices):
+            ch = C.c_void_p()
+            _check(lib().lbffs_batch_context(self._h, i, C.byref(ch)), "lbffs_batch_context")
+            self.lattice.append(Lattice._view(ch, nx, ny, nz, q, storage))
+        self.N = sum(l.N for l in self.lattice)
+
+    def set_launch(self, grid=0, graph_chunk=0):
+        _check(lib().lbffs_batch_set_launch(self._h, int(grid), int(graph_chunk)), "lbffs_batch_set_launch")
+
+    def step(self, nsteps, params):
+        """params: one dict of step scalars per lattice (or one dict for all)."""
+        n = len(self.lattice)
+        plist = params if isinstance(params, (list, tuple)) else [params] * n
+        arr = (StepParams * n)(*[_params(**p) for p in plist])
+        _check(lib().lbffs_batch_step(self._h, int(nsteps), arr), "lbffs_batch_step")
+
+    def last_step_ms(self):
+        ms = C.c_float()
+        _check(lib().lbffs_batch_last_step_ms(self._h, C.byref(ms)), "lbffs_batch_last_step_ms")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for l in self.lattice:
+                l._h = None
+            lib().lbffs_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -219,3 +219,22 @@ def backward_step(nx=1536, ny=192, nz=128, H=64, step_len=None, u_in=0.05, turb=
     sdf[near] = d[near].astype(np.float32)
     c.meta.update(H=H, step_len=step_len, u_in=u_in)
     return c
+
+
+def ensemble(n_domains=512, seed=SEED, edges=(24, 96), re_range=(50.0, 400.0), sgs=SGS_WALE, q=19):
+    """Independent cubic domains for a batch: lid-driven cavities and cylinder channels in
+    turn, edge length uniform in [edges[0], edges[1]] and Reynolds number uniform in
+    re_range, drawn from a seeded generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for d in range(n_domains):
+        n = int(rng.integers(edges[0], edges[1] + 1))
+        Re = float(rng.uniform(*re_range))
+        if d % 2 == 0:
+            c = cavity(n, Re=Re, sgs=sgs, q=q)
+        else:
+            c = cylinder_channel(nx=n, ny=n, nz=n, D=max(4, n // 5), cx=n // 4, cy=n // 2, Re=Re,
+                                 spz_cells=max(2, n // 10), sgs=sgs, seed=seed + d, q=q)
+        c.meta.update(edge=n, Re=Re)
+        out.append(c)
+    return out

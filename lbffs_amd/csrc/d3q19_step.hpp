@@ -622,12 +622,10 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
     }
 }
 
+// One cell update (the body of k_streamingCollision for cell icu < a.N).
 template <typename S, typename Real, int SGS>
-__global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
+__device__ __forceinline__ void step19_cell(const Step19Args<Real> &a, const uint32_t icu)
 {
-    const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t icu = blk * blockDim.x + threadIdx.x;
-    if (icu >= a.N) return;
     const uint32_t fl = a.flag[icu];
     if (fl & F_SOLID) return;
     const long ic = icu;
@@ -729,6 +727,15 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
         for (int q = 0; q < 19; q++)
             Codec<S, Real, 19>::store((S *)((char *)(fd + (size_t)q * N) + doff), out[q], q, w19<Real>(q));
     }
+}
+
+template <typename S, typename Real, int SGS>
+__global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
+{
+    const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t icu = blk * blockDim.x + threadIdx.x;
+    if (icu >= a.N) return;
+    step19_cell<S, Real, SGS>(a, icu);
 }
 
 }  // namespace lbffs

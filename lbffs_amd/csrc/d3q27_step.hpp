@@ -493,12 +493,10 @@ __device__ inline void cumulant27(Real *ft, Real rho, Real u, Real v, Real w, Re
     for (int q = 0; q < 27; q++) ft[q] = K[L27::slot[q]];
 }
 
+// One cell update (the body of k_streamingCollision for cell icu < a.N).
 template <typename S, typename Real, int SGS>
-__global__ void __launch_bounds__(256) step27_kernel(const StepArgs<Real, 27> a)
+__device__ __forceinline__ void step27_cell(const StepArgs<Real, 27> &a, const uint32_t icu)
 {
-    const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t icu = blk * blockDim.x + threadIdx.x;
-    if (icu >= a.N) return;
     const uint32_t fl = a.flag[icu];
     if (fl & F_SOLID) return;
     const long ic = icu;
@@ -621,6 +619,15 @@ __global__ void __launch_bounds__(256) step27_kernel(const StepArgs<Real, 27> a)
         for (int q = 0; q < 27; q++)
             Codec<S, Real, 27>::store((S *)((char *)(fd + (size_t)q * N) + doff), ft[q], q, w27<Real>(q));
     }
+}
+
+template <typename S, typename Real, int SGS>
+__global__ void __launch_bounds__(256) step27_kernel(const StepArgs<Real, 27> a)
+{
+    const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t icu = blk * blockDim.x + threadIdx.x;
+    if (icu >= a.N) return;
+    step27_cell<S, Real, SGS>(a, icu);
 }
 
 }  // namespace lbffs

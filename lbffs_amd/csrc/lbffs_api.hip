@@ -13,6 +13,7 @@
 #include "d3q19_step.hpp"
 #include "d3q27_step.hpp"
 #include "ibm.hpp"
+#include "batch.hpp"
 
 using namespace lbffs;
 
@@ -68,6 +69,7 @@ struct lbffs_ctx {
     bool f_loaded = false;
     int sponge_mask = 0;
     hipStream_t stream = nullptr;
+    bool owns_stream = true;  // false for the lattices of a batch, which share its stream
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     int block = 256;  // work-group size of the step launch
@@ -619,7 +621,145 @@ void free_all(lbffs_ctx *c)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream && c->owns_stream) hipStreamDestroy(c->stream);
+}
+
+}  // namespace
+
+struct lbffs_batch {
+    std::vector<lbffs_ctx *> ctx;
+    hipStream_t stream = nullptr;
+    void *args_dev[2] = {nullptr, nullptr};  // launch arguments of every lattice, per step parity
+    std::vector<char> args_host[2];
+    BatchBlock *table = nullptr;
+    uint32_t nblocks = 0;
+    int grid = 0;        // work-groups of the launch; 0 = one per table entry
+    int graph_chunk = 0; // steps per captured hipGraph; 0 = plain launches
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int gexec_chunk[2] = {0, 0};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+};
+
+namespace {
+
+void batch_drop_graphs(lbffs_batch *b)
+{
+    for (int p = 0; p < 2; p++) {
+        if (b->gexec[p]) (void)hipGraphExecDestroy(b->gexec[p]);
+        b->gexec[p] = nullptr;
+        b->gexec_chunk[p] = 0;
+    }
+}
+
+template <typename S, typename Real, int Q, int SGS, int LAST>
+void batch_launch(lbffs_batch *b, int parity)
+{
+    const unsigned grid = b->grid > 0 ? (unsigned)b->grid : b->nblocks;
+    hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, LAST>), dim3(grid), dim3(kBatchBlock), 0, b->stream,
+                       (const StepArgs<Real, Q> *)b->args_dev[parity], (const BatchBlock *)b->table, b->nblocks);
+}
+
+template <typename S, typename Real, int Q, int SGS>
+int batch_steps_sgs(lbffs_batch *b, int nsteps, int cur0)
+{
+    int s = 0;
+    // all steps but the last, in captured chunks when asked
+    if (b->graph_chunk > 0) {
+        const int chunk = b->graph_chunk;
+        while (nsteps - 1 - s >= chunk) {
+            const int p = (cur0 + s) & 1;
+            if (!b->gexec[p] || b->gexec_chunk[p] != chunk) {
+                if (b->gexec[p]) (void)hipGraphExecDestroy(b->gexec[p]);
+                b->gexec[p] = nullptr;
+                hipGraph_t g = nullptr;
+                HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                for (int k = 0; k < chunk; k++) batch_launch<S, Real, Q, SGS, 0>(b, (p + k) & 1);
+                HIP_TRY(hipStreamEndCapture(b->stream, &g));
+                hipError_t e = hipGraphInstantiate(&b->gexec[p], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) return fail(LBFFS_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+                b->gexec_chunk[p] = chunk;
+            }
+            HIP_TRY(hipGraphLaunch(b->gexec[p], b->stream));
+            s += chunk;
+        }
+    }
+    for (; s < nsteps; s++) {
+        const int p = (cur0 + s) & 1;
+        if (s == nsteps - 1) batch_launch<S, Real, Q, SGS, 1>(b, p);
+        else batch_launch<S, Real, Q, SGS, 0>(b, p);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+template <typename S, typename Real, int Q>
+int batch_steps_q(lbffs_batch *b, int nsteps, const lbffs_step_params *params)
+{
+    const int cur0 = b->ctx[0]->cur;
+    for (int p = 0; p < 2; p++) {
+        b->args_host[p].assign(b->ctx.size() * sizeof(StepArgs<Real, Q>), 0);
+        StepArgs<Real, Q> *A = (StepArgs<Real, Q> *)b->args_host[p].data();
+        for (size_t d = 0; d < b->ctx.size(); d++) {
+            lbffs_ctx *c = b->ctx[d];
+            StepArgs<Real, Q> &a = A[d];
+            set_args<S, Real, Q>(c, a, &params[d]);
+            a.fsrc = c->f[p];
+            a.fdst = c->f[1 - p];
+            a.rho_o = (const Real *)c->rho[p];
+            a.u_o = (const Real *)c->u[p];
+            a.v_o = (const Real *)c->v[p];
+            a.w_o = (const Real *)c->w[p];
+            a.rho_n = (Real *)c->rho[1 - p];
+            a.u_n = (Real *)c->u[1 - p];
+            a.v_n = (Real *)c->v[1 - p];
+            a.w_n = (Real *)c->w[1 - p];
+            a.aux_all = params[d].store_every ? 1 : 0;
+            a.mac_all = (a.aux_all || c->d.sgs == LBFFS_SGS_WALE) ? 1 : 0;
+        }
+        HIP_TRY(hipMemcpy(b->args_dev[p], b->args_host[p].data(), b->args_host[p].size(), hipMemcpyHostToDevice));
+    }
+    switch (b->ctx[0]->d.sgs) {
+    case LBFFS_SGS_WALE: return batch_steps_sgs<S, Real, Q, 0>(b, nsteps, cur0);
+    case LBFFS_SGS_SMAGORINSKY: return batch_steps_sgs<S, Real, Q, 1>(b, nsteps, cur0);
+    default: return batch_steps_sgs<S, Real, Q, 2>(b, nsteps, cur0);
+    }
+}
+
+template <typename S, typename Real>
+struct BatchStepOp {
+    static int run(lbffs_batch *b, int n, const lbffs_step_params *p)
+    {
+        return b->ctx[0]->d.q == 27 ? batch_steps_q<S, Real, 27>(b, n, p) : batch_steps_q<S, Real, 19>(b, n, p);
+    }
+};
+
+template <template <typename, typename> class F, typename... A>
+int batch_dispatch(lbffs_batch *b, A... args)
+{
+    switch (b->ctx[0]->d.storage) {
+    case LBFFS_STORE_F32: return F<float, float>::run(b, args...);
+    case LBFFS_STORE_F16DEV: return F<__half, float>::run(b, args...);
+    case LBFFS_STORE_BF16DEV: return F<bf16_t, float>::run(b, args...);
+    case LBFFS_STORE_F64: return F<double, double>::run(b, args...);
+    }
+    return fail(LBFFS_E_UNSUPPORTED, "storage form");
+}
+
+void batch_free(lbffs_batch *b)
+{
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graphs(b);
+    for (lbffs_ctx *c : b->ctx) {
+        free_all(c);
+        delete c;
+    }
+    for (void *p : {b->args_dev[0], b->args_dev[1], (void *)b->table})
+        if (p) (void)hipFree(p);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
 }  // namespace
@@ -957,5 +1097,121 @@ const char *lbffs_strerror(int code)
 }
 
 const char *lbffs_last_error(void) { return g_last_error.c_str(); }
+
+int lbffs_batch_create(const lbffs_desc *descs, int n, lbffs_batch **out)
+{
+    if (!descs || !out) return fail(LBFFS_E_INVALID, "null argument");
+    *out = nullptr;
+    if (n < 1) return fail(LBFFS_E_INVALID, "a batch needs at least one lattice");
+    for (int d = 1; d < n; d++)
+        if (descs[d].q != descs[0].q || descs[d].storage != descs[0].storage || descs[d].sgs != descs[0].sgs ||
+            descs[d].collision != descs[0].collision)
+            return fail(LBFFS_E_INVALID, "the lattices of a batch share q, storage, sgs and collision");
+    lbffs_batch *b = new lbffs_batch();
+    hipError_t he = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreate(&b->ev0);
+    if (he == hipSuccess) he = hipEventCreate(&b->ev1);
+    if (he != hipSuccess) {
+        batch_free(b);
+        delete b;
+        return fail(LBFFS_E_DEVICE, std::string("batch stream: ") + hipGetErrorString(he));
+    }
+    std::vector<BatchBlock> table;
+    for (int d = 0; d < n; d++) {
+        lbffs_ctx *c = nullptr;
+        int e = lbffs_create(&descs[d], &c);
+        if (e) {
+            std::string m = g_last_error;
+            batch_free(b);
+            delete b;
+            return fail(e, "lattice " + std::to_string(d) + ": " + m);
+        }
+        (void)hipStreamDestroy(c->stream);
+        c->stream = b->stream;
+        c->owns_stream = false;
+        b->ctx.push_back(c);
+        for (long first = 0; first < c->N; first += kBatchBlock) table.push_back({(uint32_t)d, (uint32_t)first});
+    }
+    b->nblocks = (uint32_t)table.size();
+    const size_t amax = sizeof(StepArgs<double, 27>) * (size_t)n;
+    int e = dalloc(&b->args_dev[0], amax);
+    if (!e) e = dalloc(&b->args_dev[1], amax);
+    if (!e) e = dalloc((void **)&b->table, table.size() * sizeof(BatchBlock));
+    if (!e && hipMemcpy(b->table, table.data(), table.size() * sizeof(BatchBlock), hipMemcpyHostToDevice) != hipSuccess)
+        e = fail(LBFFS_E_DEVICE, "batch table upload");
+    if (e) {
+        batch_free(b);
+        delete b;
+        return e;
+    }
+    *out = b;
+    return 0;
+}
+
+int lbffs_batch_context(lbffs_batch *b, int i, lbffs_ctx **ctx)
+{
+    if (!b || !ctx) return fail(LBFFS_E_INVALID, "null argument");
+    if (i < 0 || i >= (int)b->ctx.size()) return fail(LBFFS_E_INVALID, "lattice index");
+    *ctx = b->ctx[i];
+    return 0;
+}
+
+int lbffs_batch_set_launch(lbffs_batch *b, int grid, int graph_chunk)
+{
+    if (!b) return fail(LBFFS_E_INVALID, "null argument");
+    if (grid < 0 || graph_chunk < 0 || graph_chunk > 4096) return fail(LBFFS_E_INVALID, "grid/graph_chunk");
+    b->grid = grid;
+    b->graph_chunk = graph_chunk;
+    batch_drop_graphs(b);
+    return 0;
+}
+
+int lbffs_batch_step(lbffs_batch *b, int nsteps, const lbffs_step_params *params)
+{
+    if (!b || !params) return fail(LBFFS_E_INVALID, "null argument");
+    if (nsteps < 0) return fail(LBFFS_E_INVALID, "nsteps < 0");
+    HIP_TRY(hipStreamSynchronize(b->stream));  // the argument arrays are rewritten below
+    const int cur0 = b->ctx[0]->cur;
+    for (size_t d = 0; d < b->ctx.size(); d++) {
+        lbffs_ctx *c = b->ctx[d];
+        if (!c->f_loaded) return fail(LBFFS_E_STATE, "lattice " + std::to_string(d) + ": populations never uploaded");
+        if (ibm_active(c)) return fail(LBFFS_E_UNSUPPORTED, "moving-wall markers in a batch");
+        if (c->cur != cur0 || c->mcur != cur0)
+            return fail(LBFFS_E_STATE, "lattice " + std::to_string(d) + " was stepped on its own: step parity differs");
+        if (c->flags_dirty) {
+            int e = is_f64(c) ? pack_flags<double>(c) : pack_flags<float>(c);
+            if (e) return fail(e, "lattice " + std::to_string(d) + ": " + g_last_error);
+        }
+    }
+    if (nsteps == 0) return 0;
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    int e = batch_dispatch<BatchStepOp>(b, nsteps, params);
+    if (e) return e;
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    for (lbffs_ctx *c : b->ctx) {
+        c->cur ^= (nsteps & 1);
+        c->mcur ^= (nsteps & 1);
+        c->timed = false;
+    }
+    b->timed = true;
+    return 0;
+}
+
+int lbffs_batch_last_step_ms(lbffs_batch *b, float *ms)
+{
+    if (!b || !ms) return fail(LBFFS_E_INVALID, "null argument");
+    if (!b->timed) return fail(LBFFS_E_STATE, "no step has run");
+    HIP_TRY(hipEventSynchronize(b->ev1));
+    HIP_TRY(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return 0;
+}
+
+int lbffs_batch_destroy(lbffs_batch *b)
+{
+    if (!b) return 0;
+    batch_free(b);
+    delete b;
+    return 0;
+}
 
 }  // extern "C"

@@ -48,6 +48,15 @@ def test_strerror_and_invalid_arguments():
     assert L.lbffs_ibm_set_markers(None, None, 0) == -1
     assert L.lbffs_ibm_set_motion(None, None) == -1
     assert L.lbffs_ibm_download(None, 0, None, 0) == -1
+    # a batch shares the lattice type (validated before any device call)
+    descs = (lb.Desc * 2)(lb.Desc(abi_version=lb.ABI_VERSION, q=19, nx=8, ny=8, nz=8, omega0=1.0),
+                          lb.Desc(abi_version=lb.ABI_VERSION, q=27, nx=8, ny=8, nz=8, omega0=1.0,
+                                  collision=lb.COLL_CUMULANT))
+    assert L.lbffs_batch_create(descs, 2, ctypes.byref(h)) == -1
+    assert b"share" in L.lbffs_last_error()
+    assert L.lbffs_batch_create(descs, 0, ctypes.byref(h)) == -1
+    assert L.lbffs_batch_step(None, 1, None) == -1
+    assert L.lbffs_batch_destroy(None) == 0
     assert L.lbffs_upload(None, 0, None, 0) == -1
     assert L.lbffs_step(None, 1, None) == -1
     assert L.lbffs_destroy(None) == 0
