@@ -221,20 +221,43 @@ def backward_step(nx=1536, ny=192, nz=128, H=64, step_len=None, u_in=0.05, turb=
     return c
 
 
-def ensemble(n_domains=512, seed=SEED, edges=(24, 96), re_range=(50.0, 400.0), sgs=SGS_WALE, q=19):
-    """Independent cubic domains for a batch: lid-driven cavities and cylinder channels in
-    turn, edge length uniform in [edges[0], edges[1]] and Reynolds number uniform in
-    re_range, drawn from a seeded generator."""
+def ensemble_specs(n_domains=512, seed=SEED, edges=(24, 96), re_range=(50.0, 400.0)):
+    """(kind, edge, Re) of the independent cubic domains of a batch: lid-driven cavities and
+    cylinder channels in turn, edge uniform in [edges[0], edges[1]], Reynolds number
+    uniform in re_range, from a seeded generator."""
     rng = np.random.default_rng(seed)
     out = []
     for d in range(n_domains):
         n = int(rng.integers(edges[0], edges[1] + 1))
         Re = float(rng.uniform(*re_range))
-        if d % 2 == 0:
-            c = cavity(n, Re=Re, sgs=sgs, q=q)
-        else:
-            c = cylinder_channel(nx=n, ny=n, nz=n, D=max(4, n // 5), cx=n // 4, cy=n // 2, Re=Re,
-                                 spz_cells=max(2, n // 10), sgs=sgs, seed=seed + d, q=q)
-        c.meta.update(edge=n, Re=Re)
-        out.append(c)
+        out.append(("cavity" if d % 2 == 0 else "cylinder", n, Re))
     return out
+
+
+def ensemble_case(spec, index=0, seed=SEED, sgs=SGS_WALE, q=19):
+    kind, n, Re = spec
+    if kind == "cavity":
+        c = cavity(n, Re=Re, sgs=sgs, q=q)
+    else:
+        c = cylinder_channel(nx=n, ny=n, nz=n, D=max(4, n // 5), cx=n // 4, cy=n // 2, Re=Re,
+                             spz_cells=max(2, n // 10), sgs=sgs, seed=seed + index, q=q)
+    c.meta.update(edge=n, Re=Re)
+    return c
+
+
+def ensemble_omega0(spec):
+    """omega0 of ensemble_case(spec) without building its arrays (the builders' units)."""
+    kind, n, Re = spec
+    if kind == "cavity":
+        U0 = 0.1
+        nu = (U0 * n / Re) * (1.0 / (n - 1)) * (1.0 / U0)
+    else:
+        U0 = 0.05
+        nu = (U0 * max(4, n // 5) / Re) * (1.0 / (n - 1)) * (1.0 / U0)
+    _, un = units(n, n, n, 1.0, 1.0, U0, nu)
+    return float(un.omega0)
+
+
+def ensemble(n_domains=512, seed=SEED, edges=(24, 96), re_range=(50.0, 400.0), sgs=SGS_WALE, q=19):
+    """The domains of ensemble_specs, built."""
+    return [ensemble_case(sp, d, seed, sgs, q) for d, sp in enumerate(ensemble_specs(n_domains, seed, edges, re_range))]

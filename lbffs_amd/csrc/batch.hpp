@@ -19,23 +19,28 @@ struct BatchBlock {
 
 constexpr int kBatchBlock = 256;
 
-// LAST = 1 on the last step of a call: every cell stores its moments and diagnostics
-template <typename S, typename Real, int SGS, int Q, int LAST>
+// The launch arguments of every lattice are read from device memory where they are used
+// (uniform per work-group, so through scalar loads); the host prepares a separate argument
+// array for the last step of a call, whose cells store every moment and diagnostic.
+template <typename S, typename Real, int SGS, int Q>
 __global__ void __launch_bounds__(kBatchBlock)
     batch_step_kernel(const StepArgs<Real, Q> *__restrict__ args, const BatchBlock *__restrict__ table,
                       uint32_t nblocks)
 {
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         const BatchBlock e = table[b];
-        StepArgs<Real, Q> a = args[e.lattice];
+        const StepArgs<Real, Q> &a = args[e.lattice];
         const uint32_t icu = e.first + threadIdx.x;
         if (icu >= a.N) continue;
-        if (LAST) {
-            a.aux_all = 1;
-            a.mac_all = 1;
-        }
-        if constexpr (Q == 27) step27_cell<S, Real, SGS>(a, icu);
-        else step19_cell<S, Real, SGS>(a, icu);
+        do {
+#define CELL_RETURN continue
+            if constexpr (Q == 27) {
+#include "d3q27_cell.inc"
+            } else {
+#include "d3q19_cell.inc"
+            }
+#undef CELL_RETURN
+        } while (0);
     }
 }
 

@@ -622,120 +622,15 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
     }
 }
 
-// One cell update (the body of k_streamingCollision for cell icu < a.N).
-template <typename S, typename Real, int SGS>
-__device__ __forceinline__ void step19_cell(const Step19Args<Real> &a, const uint32_t icu)
-{
-    const uint32_t fl = a.flag[icu];
-    if (fl & F_SOLID) return;
-    const long ic = icu;
-    const uint32_t N = a.N;
-    const uint32_t t = fdiv(icu, a.dnx);
-    const int i = (int)(icu - t * (uint32_t)a.nx);
-    const uint32_t kk = fdiv(t, a.dny);
-    const int j = (int)(t - kk * (uint32_t)a.ny);
-    const int k = (int)kk;
-    const S *fs = (const S *)a.fsrc;
-    S *fd = (S *)a.fdst;
-
-    const Real omega = (fl & F_OMEGAVAR) ? a.omega[ic] : a.omega0;
-    Real ft[19];
-    if (!(fl & F_FACE)) {
-        // interior pull: uniform per-direction base (SGPRs) + 32-bit byte offset (VGPR),
-        // so each load is one global_load with an SGPR base
-        // (the offset is formed from the compile-time lattice vector, so the three pulls
-        // that share a y/z shift differ only by an immediate)
-        const uint32_t boff = icu * (uint32_t)sizeof(S);
-        const uint32_t sy = (uint32_t)a.nx * (uint32_t)sizeof(S);
-        const uint32_t sz = (a.nz == 1) ? 0u : (uint32_t)a.nx * (uint32_t)a.ny * (uint32_t)sizeof(S);
-#pragma unroll
-        for (int q = 0; q < 19; q++) {
-            const char *fq = (const char *)(fs + (size_t)q * N);
-            const uint32_t yz = boff - (uint32_t)L19::cy[q] * sy - (uint32_t)L19::cz[q] * sz;
-            const uint32_t src = yz - (uint32_t)(L19::cx[q] * (int)sizeof(S));
-            ft[q] = Codec<S, Real, 19>::load((const S *)(fq + src), q, w19<Real>(q));
-        }
-    } else {
-        face_cell<S, Real>(a, ft, fl, ic, i, j, k, omega);
-    }
-
-    // internalWallBC, kernel.cl:1926-2003: simple bounce-back on solid links
-    Real Fx = 0, Fy = 0, Fz = 0;
-    if (fl & F_NEISOLID) {
-        const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
-#pragma unroll
-        for (int q = 1; q < 19; q++) {
-            const long src = (fl & F_FACE) ? face_upwind(q, i, j, k, a.nx, a.ny, a.nz, b1, b2, b3) : ic - a.off[q];
-            if (src >= 0 && (a.flag[src] & F_SOLID)) {
-                const int qb = L19::opp(q);
-                const Real fb = Codec<S, Real, 19>::load(fs + (size_t)qb * N + icu, qb, w19<Real>(qb));
-                ft[q] = fb;
-                Fx += -(fb + fb) * Real(L19::cx[q]);
-                Fy += -(fb + fb) * Real(L19::cy[q]);
-                Fz += -(fb + fb) * Real(L19::cz[q]);
-            }
-        }
-    }
-
-    // updateRhoUVW, kernel.cl:3509-3523 (u carries the half-step force shift)
-    const Mom19<Real> m = moments19<Real>(ft);
-    const Real rho = m.rho;
-    const Real inv_rho = rcp_nr(rho);
-    const Real hf = Real(0.5) * a.dpdx * inv_rho;
-    const Real u = m.mx * inv_rho + hf;
-    const Real v = m.my * inv_rho, w = m.mz * inv_rho;
-    if (a.mac_all || (fl & F_KEEPMAC)) {
-        a.rho_n[ic] = rho;
-        a.u_n[ic] = u;
-        a.v_n[ic] = v;
-        a.w_n[ic] = w;
-    }
-    if (a.aux_all || (fl & F_NEISOLID)) {
-        a.Fwx[ic] = Fx;
-        a.Fwy[ic] = Fy;
-        a.Fwz[ic] = Fz;
-    }
-    if (a.aux_all) {
-        a.Gx[ic] = Real(0);
-        a.Gy[ic] = Real(0);
-        a.Gz[ic] = Real(0);
-    }
-
-    const Real tau0 = rcp(omega);
-    Real tauSGS;
-    if constexpr (SGS == 0) {
-        tauSGS = wale_tau<Real>(a, ic, i, j, k, u, v, w);
-    } else if constexpr (SGS == 1) {
-        const Real sdf = (fl & F_HASSDF) ? a.sdf[ic] : Real(10000);
-        tauSGS = smagorinsky_tau<Real>(m, u, v, w, tau0, inv_rho, a.LES, sdf, a.Cs, a.damping);
-    } else {
-        tauSGS = Real(0);
-    }
-    if (a.aux_all) a.tau[ic] = tauSGS;
-
-    const Real tau = sponge_tau<Real>(a, tau0, tauSGS, i, j, k);
-    const Real omegaEff = rcp(tau + tauSGS);
-    Real out[19];
-    collide_rr<Real>(out, m, inv_rho, u - hf, v, w, omegaEff, a.omegaB, a.dpdx);
-    const uint32_t doff = icu * (uint32_t)sizeof(S);
-    if (a.tune & TUNE_NT_STORE) {
-#pragma unroll
-        for (int q = 0; q < 19; q++)
-            Codec<S, Real, 19>::store_nt((S *)((char *)(fd + (size_t)q * N) + doff), out[q], q, w19<Real>(q));
-    } else {
-#pragma unroll
-        for (int q = 0; q < 19; q++)
-            Codec<S, Real, 19>::store((S *)((char *)(fd + (size_t)q * N) + doff), out[q], q, w19<Real>(q));
-    }
-}
-
 template <typename S, typename Real, int SGS>
 __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
 {
     const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     const uint32_t icu = blk * blockDim.x + threadIdx.x;
     if (icu >= a.N) return;
-    step19_cell<S, Real, SGS>(a, icu);
+#define CELL_RETURN return
+#include "d3q19_cell.inc"
+#undef CELL_RETURN
 }
 
 }  // namespace lbffs

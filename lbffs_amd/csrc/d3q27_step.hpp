@@ -493,141 +493,15 @@ __device__ inline void cumulant27(Real *ft, Real rho, Real u, Real v, Real w, Re
     for (int q = 0; q < 27; q++) ft[q] = K[L27::slot[q]];
 }
 
-// One cell update (the body of k_streamingCollision for cell icu < a.N).
-template <typename S, typename Real, int SGS>
-__device__ __forceinline__ void step27_cell(const StepArgs<Real, 27> &a, const uint32_t icu)
-{
-    const uint32_t fl = a.flag[icu];
-    if (fl & F_SOLID) return;
-    const long ic = icu;
-    const uint32_t N = a.N;
-    const uint32_t t = fdiv(icu, a.dnx);
-    const int i = (int)(icu - t * (uint32_t)a.nx);
-    const uint32_t kk = fdiv(t, a.dny);
-    const int j = (int)(t - kk * (uint32_t)a.ny);
-    const int k = (int)kk;
-    const S *fs = (const S *)a.fsrc;
-    S *fd = (S *)a.fdst;
-
-    const Real omega = (fl & F_OMEGAVAR) ? a.omega[ic] : a.omega0;
-    Real ft[27];
-    if (!(fl & F_FACE)) {
-        // interior pull: per-direction SGPR base + 32-bit byte offset; a 2-D lattice
-        // keeps the z links in the layer (K27:797-820)
-        const uint32_t boff = icu * (uint32_t)sizeof(S);
-        const uint32_t sy = (uint32_t)a.nx * (uint32_t)sizeof(S);
-        const uint32_t sz = (a.nz == 1) ? 0u : (uint32_t)a.nx * (uint32_t)a.ny * (uint32_t)sizeof(S);
-#pragma unroll
-        for (int q = 0; q < 27; q++) {
-            const char *fq = (const char *)(fs + (size_t)q * N);
-            const uint32_t yz = boff - (uint32_t)L27::cy[q] * sy - (uint32_t)L27::cz[q] * sz;
-            const uint32_t src = yz - (uint32_t)(L27::cx[q] * (int)sizeof(S));
-            ft[q] = Codec<S, Real, 27>::load((const S *)(fq + src), q, w27<Real>(q));
-        }
-    } else {
-        face_cell27<S, Real>(a, ft, fl, ic, i, j, k, omega);
-    }
-
-    // internalWallBC, K27:1823-1897: interpolated bounce-back on solid links with the
-    // cell's moments and tauSGS of the previous step; q from the signed distances.
-    // A NEISOLID face cell lies only on faces that wrap (the flags refuse one on a wall
-    // face, where the reference reads solid[-1]), so every link below has a source.
-    Real Fx = 0, Fy = 0, Fz = 0;
-    if (fl & F_NEISOLID) {
-        const bool face = (fl & F_FACE) != 0;
-        const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
-        const Real rhoo = a.rho_o[ic], uo = a.u_o[ic], vo = a.v_o[ic], wo = a.w_o[ic];
-        const Real omegaW = Real(1) / (Real(1) / omega + a.tau[ic]);
-        const Real uSqr = uo * uo + vo * vo + wo * wo;
-        const Real sdf0 = fabs(a.sdf[ic]);
-#pragma unroll
-        for (int q = 1; q < 27; q++) {
-            const int qb = L27::opp(q);
-            const long src = face ? face_upwind27(q, i, j, k, a.nx, a.ny, a.nz, b1, b2, b3) : ic - a.off[q];
-            if (src >= 0 && (a.flag[src] & F_SOLID)) {
-                const Real fb = Codec<S, Real, 27>::load(fs + (size_t)qb * N + icu, qb, w27<Real>(qb));
-                const Real cx = Real(L27::cx[q]), cy = Real(L27::cy[q]), cz = Real(L27::cz[q]);
-                const Real qf = sdf0 / (sdf0 + fabs(a.sdf[src]));
-                const Real uDotC = -uo * cx - vo * cy - wo * cz;
-                Real feq, chi;
-                if (qf <= Real(0.5)) {
-                    // the fluid cell beyond, upID[qbb] = x + c_q
-                    long nb = face ? face_upwind27(qb, i, j, k, a.nx, a.ny, a.nz, b1, b2, b3) : ic + a.off[q];
-                    nb = nb < 0 ? ic : nb;
-                    const Real ufDotC = -a.u_o[nb] * cx - a.v_o[nb] * cy - a.w_o[nb] * cz;
-                    feq = (rhoo + Real(3) * ufDotC + Real(4.5) * uDotC * uDotC - Real(1.5) * uSqr) * w27<Real>(q);
-                    chi = omegaW * (Real(2) * qf - Real(1)) / (Real(1) - Real(2) * omegaW);
-                } else {
-                    feq = (rhoo + Real(3) * (Real(1) - Real(1) / qf) * uDotC + Real(4.5) * uDotC * uDotC
-                           - Real(1.5) * uSqr) * w27<Real>(q);
-                    chi = omegaW * (Real(2) * qf - Real(1));
-                }
-                ft[q] = (Real(1) - chi) * fb + chi * feq;
-                Fx += -(fb + ft[q]) * cx;
-                Fy += -(fb + ft[q]) * cy;
-                Fz += -(fb + ft[q]) * cz;
-            }
-        }
-    }
-
-    // updateRhoUVW (u carries the half-step force shift)
-    const Mom19<Real> m = moments27<Real>(ft);
-    const Real rho = m.rho;
-    const Real inv_rho = rcp_nr(rho);
-    const Real u = m.mx * inv_rho + Real(0.5) * a.dpdx * inv_rho;
-    const Real v = m.my * inv_rho, w = m.mz * inv_rho;
-    if (a.mac_all || (fl & F_KEEPMAC)) {
-        a.rho_n[ic] = rho;
-        a.u_n[ic] = u;
-        a.v_n[ic] = v;
-        a.w_n[ic] = w;
-    }
-    if (a.aux_all || (fl & F_NEISOLID)) {
-        a.Fwx[ic] = Fx;
-        a.Fwy[ic] = Fy;
-        a.Fwz[ic] = Fz;
-    }
-    if (a.aux_all) {
-        a.Gx[ic] = Real(0);
-        a.Gy[ic] = Real(0);
-        a.Gz[ic] = Real(0);
-    }
-
-    const Real tau0 = rcp(omega);
-    Real tauSGS;
-    if constexpr (SGS == 0) {
-        tauSGS = wale_tau<Real>(a, ic, i, j, k, u, v, w);
-    } else if constexpr (SGS == 1) {
-        const Real sdf = (fl & F_HASSDF) ? a.sdf[ic] : Real(10000);
-        tauSGS = smagorinsky_tau<Real>(m, u, v, w, tau0, inv_rho, a.LES, sdf, a.Cs, a.damping);
-    } else {
-        tauSGS = Real(0);
-    }
-    // the interpolated bounce-back of the next step reads this cell's tauSGS
-    if (a.aux_all || (fl & F_NEISOLID)) a.tau[ic] = tauSGS;
-
-    const Real tau = sponge_tau<Real>(a, tau0, tauSGS, i, j, k);
-    const Real omegaEff = rcp(tau + tauSGS);
-    cumulant27<Real>(ft, rho, u, v, w, omegaEff, a.omegaB, a.clim);
-    const uint32_t doff = icu * (uint32_t)sizeof(S);
-    if (a.tune & TUNE_NT_STORE) {
-#pragma unroll
-        for (int q = 0; q < 27; q++)
-            Codec<S, Real, 27>::store_nt((S *)((char *)(fd + (size_t)q * N) + doff), ft[q], q, w27<Real>(q));
-    } else {
-#pragma unroll
-        for (int q = 0; q < 27; q++)
-            Codec<S, Real, 27>::store((S *)((char *)(fd + (size_t)q * N) + doff), ft[q], q, w27<Real>(q));
-    }
-}
-
 template <typename S, typename Real, int SGS>
 __global__ void __launch_bounds__(256) step27_kernel(const StepArgs<Real, 27> a)
 {
     const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     const uint32_t icu = blk * blockDim.x + threadIdx.x;
     if (icu >= a.N) return;
-    step27_cell<S, Real, SGS>(a, icu);
+#define CELL_RETURN return
+#include "d3q27_cell.inc"
+#undef CELL_RETURN
 }
 
 }  // namespace lbffs

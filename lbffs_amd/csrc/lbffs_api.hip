@@ -629,8 +629,9 @@ void free_all(lbffs_ctx *c)
 struct lbffs_batch {
     std::vector<lbffs_ctx *> ctx;
     hipStream_t stream = nullptr;
-    void *args_dev[2] = {nullptr, nullptr};  // launch arguments of every lattice, per step parity
-    std::vector<char> args_host[2];
+    // launch arguments of every lattice: [parity + 2 * last step of a call]
+    void *args_dev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<char> args_host[4];
     BatchBlock *table = nullptr;
     uint32_t nblocks = 0;
     int grid = 0;        // work-groups of the launch; 0 = one per table entry
@@ -652,12 +653,13 @@ void batch_drop_graphs(lbffs_batch *b)
     }
 }
 
-template <typename S, typename Real, int Q, int SGS, int LAST>
-void batch_launch(lbffs_batch *b, int parity)
+template <typename S, typename Real, int Q, int SGS>
+void batch_launch(lbffs_batch *b, int parity, int last)
 {
     const unsigned grid = b->grid > 0 ? (unsigned)b->grid : b->nblocks;
-    hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, LAST>), dim3(grid), dim3(kBatchBlock), 0, b->stream,
-                       (const StepArgs<Real, Q> *)b->args_dev[parity], (const BatchBlock *)b->table, b->nblocks);
+    hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q>), dim3(grid), dim3(kBatchBlock), 0, b->stream,
+                       (const StepArgs<Real, Q> *)b->args_dev[parity + 2 * last], (const BatchBlock *)b->table,
+                       b->nblocks);
 }
 
 template <typename S, typename Real, int Q, int SGS>
@@ -674,7 +676,7 @@ int batch_steps_sgs(lbffs_batch *b, int nsteps, int cur0)
                 b->gexec[p] = nullptr;
                 hipGraph_t g = nullptr;
                 HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-                for (int k = 0; k < chunk; k++) batch_launch<S, Real, Q, SGS, 0>(b, (p + k) & 1);
+                for (int k = 0; k < chunk; k++) batch_launch<S, Real, Q, SGS>(b, (p + k) & 1, 0);
                 HIP_TRY(hipStreamEndCapture(b->stream, &g));
                 hipError_t e = hipGraphInstantiate(&b->gexec[p], g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
@@ -687,8 +689,7 @@ int batch_steps_sgs(lbffs_batch *b, int nsteps, int cur0)
     }
     for (; s < nsteps; s++) {
         const int p = (cur0 + s) & 1;
-        if (s == nsteps - 1) batch_launch<S, Real, Q, SGS, 1>(b, p);
-        else batch_launch<S, Real, Q, SGS, 0>(b, p);
+        batch_launch<S, Real, Q, SGS>(b, p, s == nsteps - 1 ? 1 : 0);
         HIP_TRY(hipGetLastError());
     }
     return 0;
@@ -698,9 +699,10 @@ template <typename S, typename Real, int Q>
 int batch_steps_q(lbffs_batch *b, int nsteps, const lbffs_step_params *params)
 {
     const int cur0 = b->ctx[0]->cur;
-    for (int p = 0; p < 2; p++) {
-        b->args_host[p].assign(b->ctx.size() * sizeof(StepArgs<Real, Q>), 0);
-        StepArgs<Real, Q> *A = (StepArgs<Real, Q> *)b->args_host[p].data();
+    for (int v = 0; v < 4; v++) {
+        const int p = v & 1, last = v >> 1;
+        b->args_host[v].assign(b->ctx.size() * sizeof(StepArgs<Real, Q>), 0);
+        StepArgs<Real, Q> *A = (StepArgs<Real, Q> *)b->args_host[v].data();
         for (size_t d = 0; d < b->ctx.size(); d++) {
             lbffs_ctx *c = b->ctx[d];
             StepArgs<Real, Q> &a = A[d];
@@ -715,10 +717,10 @@ int batch_steps_q(lbffs_batch *b, int nsteps, const lbffs_step_params *params)
             a.u_n = (Real *)c->u[1 - p];
             a.v_n = (Real *)c->v[1 - p];
             a.w_n = (Real *)c->w[1 - p];
-            a.aux_all = params[d].store_every ? 1 : 0;
+            a.aux_all = (params[d].store_every || last) ? 1 : 0;
             a.mac_all = (a.aux_all || c->d.sgs == LBFFS_SGS_WALE) ? 1 : 0;
         }
-        HIP_TRY(hipMemcpy(b->args_dev[p], b->args_host[p].data(), b->args_host[p].size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->args_dev[v], b->args_host[v].data(), b->args_host[v].size(), hipMemcpyHostToDevice));
     }
     switch (b->ctx[0]->d.sgs) {
     case LBFFS_SGS_WALE: return batch_steps_sgs<S, Real, Q, 0>(b, nsteps, cur0);
@@ -755,7 +757,7 @@ void batch_free(lbffs_batch *b)
         free_all(c);
         delete c;
     }
-    for (void *p : {b->args_dev[0], b->args_dev[1], (void *)b->table})
+    for (void *p : {b->args_dev[0], b->args_dev[1], b->args_dev[2], b->args_dev[3], (void *)b->table})
         if (p) (void)hipFree(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1134,8 +1136,8 @@ int lbffs_batch_create(const lbffs_desc *descs, int n, lbffs_batch **out)
     }
     b->nblocks = (uint32_t)table.size();
     const size_t amax = sizeof(StepArgs<double, 27>) * (size_t)n;
-    int e = dalloc(&b->args_dev[0], amax);
-    if (!e) e = dalloc(&b->args_dev[1], amax);
+    int e = 0;
+    for (int v = 0; v < 4 && !e; v++) e = dalloc(&b->args_dev[v], amax);
     if (!e) e = dalloc((void **)&b->table, table.size() * sizeof(BatchBlock));
     if (!e && hipMemcpy(b->table, table.data(), table.size() * sizeof(BatchBlock), hipMemcpyHostToDevice) != hipSuccess)
         e = fail(LBFFS_E_DEVICE, "batch table upload");
