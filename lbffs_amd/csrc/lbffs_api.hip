@@ -297,7 +297,12 @@ int dalloc(void **p, size_t bytes)
         *p = nullptr;
         return fail(LBFFS_E_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
     }
+    // hipMemset runs on the null stream and may return before it has finished; the
+    // contexts' streams are non-blocking and do not wait for it, so a kernel that writes
+    // the new buffer right away (the decode of a 16-bit download, for one) could be
+    // overwritten by the zeros: wait for the fill here.
     e = hipMemset(*p, 0, bytes);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) return fail(LBFFS_E_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
     return 0;
 }
