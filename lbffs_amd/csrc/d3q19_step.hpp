@@ -42,6 +42,7 @@ __host__ __device__ constexpr Real w19(int q)
 template <typename Real, int Q>
 struct StepArgs {
     uint32_t N;
+    uint32_t fstride;  // elements between two population arrays (>= N, padded: see the API)
     int nx, ny, nz;
     FastDiv dnx, dny;
     int off[Q];  // interior pull offset: source = ic - off[q]
@@ -482,7 +483,7 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
                                        Real omega)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const long N = a.N;
+    const long FS = a.fstride;
     const S *fs = (const S *)a.fsrc;
     const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
     const bool sym = (b1 == BC_SYMMETRY || b2 == BC_SYMMETRY || b3 == BC_SYMMETRY);
@@ -492,10 +493,10 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
     for (int q = 1; q < 19; q++) {
         long src = face_upwind(q, i, j, k, nx, ny, nz, b1, b2, b3);
         if (src >= 0) {
-            ft[q] = Codec<S, Real, 19>::load(fs + (long)q * N + src, q, w19<Real>(q));
+            ft[q] = Codec<S, Real, 19>::load(fs + (long)q * FS + src, q, w19<Real>(q));
         } else {
             int qb = sym ? mirror_q(q, b1, b2, b3) : L19::opp(q);
-            ft[q] = Codec<S, Real, 19>::load(fs + (long)qb * N + ic, qb, w19<Real>(qb));
+            ft[q] = Codec<S, Real, 19>::load(fs + (long)qb * FS + ic, qb, w19<Real>(qb));
             Fx += Real(-2) * ft[q] * Real(L19::cx[q]);
             Fy += Real(-2) * ft[q] * Real(L19::cy[q]);
             Fz += Real(-2) * ft[q] * Real(L19::cz[q]);
@@ -604,7 +605,7 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
 #pragma unroll
         for (int q = 1; q < 19; q++) {
             const int cx = L19::cx[q], cy = L19::cy[q], cz = L19::cz[q];
-            const S *fq = fs + (long)q * N;
+            const S *fq = fs + (long)q * FS;
 #define OUTF(cond, inner, un)                                                                          \
     if (cond) {                                                                                        \
         const Real fi = Codec<S, Real, 19>::load(fq + (inner), q, w19<Real>(q));                      \

@@ -208,7 +208,7 @@ __device__ __forceinline__ void face_cell27(const StepArgs<Real, 27> &a, Real *f
                                             int k, Real omega)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const long N = a.N;
+    const long FS = a.fstride;
     const S *fs = (const S *)a.fsrc;
     const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
     Real Fx = 0, Fy = 0, Fz = 0;  // wall momentum of this step's domain-face links
@@ -217,7 +217,7 @@ __device__ __forceinline__ void face_cell27(const StepArgs<Real, 27> &a, Real *f
     for (int q = 1; q < 27; q++) {
         const long src = face_upwind27(q, i, j, k, nx, ny, nz, b1, b2, b3);
         if (src >= 0) {
-            ft[q] = Codec<S, Real, 27>::load(fs + (long)q * N + src, q, w27<Real>(q));
+            ft[q] = Codec<S, Real, 27>::load(fs + (long)q * FS + src, q, w27<Real>(q));
         } else {
             // reflected population; a Symmetry face code mirrors instead, the z test
             // overriding the y test overriding the x test (K27:1133-1145)
@@ -225,7 +225,7 @@ __device__ __forceinline__ void face_cell27(const StepArgs<Real, 27> &a, Real *f
             qb = (b1 == BC_SYMMETRY) ? L27::mirx[q] : qb;
             qb = (b2 == BC_SYMMETRY) ? L27::miry[q] : qb;
             qb = (b3 == BC_SYMMETRY) ? L27::mirz[q] : qb;
-            ft[q] = Codec<S, Real, 27>::load(fs + (long)qb * N + ic, qb, w27<Real>(qb));
+            ft[q] = Codec<S, Real, 27>::load(fs + (long)qb * FS + ic, qb, w27<Real>(qb));
             Fx += Real(-2) * ft[q] * Real(L27::cx[q]);
             Fy += Real(-2) * ft[q] * Real(L27::cy[q]);
             Fz += Real(-2) * ft[q] * Real(L27::cz[q]);
@@ -261,7 +261,7 @@ __device__ __forceinline__ void face_cell27(const StepArgs<Real, 27> &a, Real *f
 #pragma unroll
         for (int q = 1; q < 27; q++) {
             const int cx = L27::cx[q], cy = L27::cy[q], cz = L27::cz[q];
-            const S *fq = fs + (long)q * N;
+            const S *fq = fs + (long)q * FS;
 #define OUTF(cond, inner, un)                                                                          \
     if (cond) {                                                                                        \
         const Real fi = Codec<S, Real, 27>::load(fq + (inner), q, w27<Real>(q));                      \
