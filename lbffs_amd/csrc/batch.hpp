@@ -23,7 +23,7 @@ constexpr int kBatchBlock = 256;
 // (uniform per work-group, so through scalar loads); the host prepares a separate argument
 // array for the last step of a call, whose cells store every moment and diagnostic.
 template <typename S, typename Real, int SGS, int Q>
-__global__ void __launch_bounds__(kBatchBlock)
+__global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Q == 27 ? kStep27MinWaves<Real> : 1, 8)))
     batch_step_kernel(const StepArgs<Real, Q> *__restrict__ args, const BatchBlock *__restrict__ table,
                       uint32_t nblocks)
 {

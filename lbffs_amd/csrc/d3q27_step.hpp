@@ -279,37 +279,55 @@ __device__ __forceinline__ void face_cell27(const StepArgs<Real, 27> &a, Real *f
     }
 }
 
-// Density, momentum and raw second moments of the 27 streamed populations.
+// First pass of the moment transform: the populations in slot order (9 z lines of 3),
+// each line reduced to its raw z moments (sum f, sum z f, sum z^2 f).  Density, z momentum
+// and the raw second moments follow from the nine lines with a few additions, so the
+// collision's central moments and the cell's moments share this pass.
 template <typename Real>
-__device__ inline Mom19<Real> moments27(const Real *ft)
+__device__ inline Mom19<Real> zpass_raw27(Real *K, const Real *ft)
 {
-    Mom19<Real> m;
-    Real r = Real(0), mx = Real(0), my = Real(0), mz = Real(0);
-    Real sxx = Real(0), syy = Real(0), szz = Real(0), sxy = Real(0), sxz = Real(0), syz = Real(0);
 #pragma unroll
-    for (int q = 0; q < 27; q++) {
-        const int x = L27::cx[q], y = L27::cy[q], z = L27::cz[q];
-        r += ft[q];
-        if (x) mx += x > 0 ? ft[q] : -ft[q];
-        if (y) my += y > 0 ? ft[q] : -ft[q];
-        if (z) mz += z > 0 ? ft[q] : -ft[q];
-        if (x) sxx += ft[q];
-        if (y) syy += ft[q];
-        if (z) szz += ft[q];
-        if (x && y) sxy += (x * y > 0) ? ft[q] : -ft[q];
-        if (x && z) sxz += (x * z > 0) ? ft[q] : -ft[q];
-        if (y && z) syz += (y * z > 0) ? ft[q] : -ft[q];
+    for (int s = 0; s < 27; s++) K[s] = ft[L27::q_of_slot[s]];
+    // x and y momentum as sums of differences of mirrored populations: at low Mach
+    // number the momentum is a small difference of order-one sums, and differencing
+    // first keeps its rounding error at the level of the reference's signed sum in
+    // direction order (opposite directions adjacent)
+    Real mx = Real(0), my = Real(0);
+#pragma unroll
+    for (int s = 0; s < 9; s++) mx += K[18 + s] - K[s];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) my += K[9 * a + 6 + c] - K[9 * a + c];
+#pragma unroll
+    for (int p = 0; p < 9; p++) {
+        const Real a = K[3 * p], b = K[3 * p + 1], c = K[3 * p + 2];
+        const Real sum = c + a;
+        K[3 * p] = sum + b;      // sum f
+        K[3 * p + 1] = c - a;    // sum z f
+        K[3 * p + 2] = sum;      // sum z^2 f
     }
-    m.rho = r;
+    // line p = 3 (x+1) + (y+1)
+#define L0(x, y) K[3 * (3 * ((x) + 1) + (y) + 1)]
+#define L1(x, y) K[3 * (3 * ((x) + 1) + (y) + 1) + 1]
+#define L2(x, y) K[3 * (3 * ((x) + 1) + (y) + 1) + 2]
+    Mom19<Real> m;
+    const Real xm = L0(-1, -1) + L0(-1, 0) + L0(-1, 1), x0 = L0(0, -1) + L0(0, 0) + L0(0, 1);
+    const Real xp = L0(1, -1) + L0(1, 0) + L0(1, 1);
+    const Real ym = L0(-1, -1) + L0(0, -1) + L0(1, -1), yp = L0(-1, 1) + L0(0, 1) + L0(1, 1);
+    m.rho = xm + x0 + xp;
     m.mx = mx;
     m.my = my;
-    m.mz = mz;
-    m.sxx = sxx;
-    m.syy = syy;
-    m.szz = szz;
-    m.sxy = sxy;
-    m.sxz = sxz;
-    m.syz = syz;
+    m.mz = (L1(-1, -1) + L1(-1, 0) + L1(-1, 1)) + (L1(0, -1) + L1(0, 0) + L1(0, 1)) + (L1(1, -1) + L1(1, 0) + L1(1, 1));
+    m.sxx = xm + xp;
+    m.syy = ym + yp;
+    m.szz = (L2(-1, -1) + L2(-1, 0) + L2(-1, 1)) + (L2(0, -1) + L2(0, 0) + L2(0, 1)) + (L2(1, -1) + L2(1, 0) + L2(1, 1));
+    m.sxy = (L0(1, 1) + L0(-1, -1)) - (L0(1, -1) + L0(-1, 1));
+    m.sxz = (L1(1, -1) + L1(1, 0) + L1(1, 1)) - (L1(-1, -1) + L1(-1, 0) + L1(-1, 1));
+    m.syz = (L1(-1, 1) + L1(0, 1) + L1(1, 1)) - (L1(-1, -1) + L1(0, -1) + L1(1, -1));
+#undef L0
+#undef L1
+#undef L2
     return m;
 }
 
@@ -327,15 +345,18 @@ __device__ inline void fwd3(Real &a, Real &b, Real &c, Real u)
     c = k2;
 }
 
-// inverse of fwd3 (K27:2563-2662): (k0, k1, k2) -> (f(-1), f(0), f(+1))
+// inverse of fwd3 (the reference's formulas, K27:2563-2662, regrouped):
+// (k0, k1, k2) -> (f(-1), f(0), f(+1)) with hu = u/2, huu = u^2/2:
+//   P = huu k0 + u k1 + k2/2,  D = hu k0 + k1/2,  f(-1) = P - D,  f(+1) = P + D,  f(0) = k0 - 2P
 template <typename Real>
-__device__ inline void bwd3(Real &a, Real &b, Real &c, Real u)
+__device__ inline void bwd3(Real &a, Real &b, Real &c, Real u, Real hu, Real huu)
 {
     const Real k0 = a, k1 = b, k2 = c;
-    const Real uu = u * u;
-    b = k0 * (Real(1) - uu) - Real(2) * u * k1 - k2;
-    a = Real(0.5) * (k0 * (uu - u) + k1 * (Real(2) * u - Real(1)) + k2);
-    c = Real(0.5) * (k0 * (uu + u) + k1 * (Real(2) * u + Real(1)) + k2);
+    const Real P = huu * k0 + (u * k1 + Real(0.5) * k2);
+    const Real D = hu * k0 + Real(0.5) * k1;
+    a = P - D;
+    c = P + D;
+    b = k0 - Real(2) * P;
 }
 
 // K27:2352-2691 collisionCumulantGeier: cumulants of the central moments about the
@@ -346,14 +367,19 @@ __device__ inline void bwd3(Real &a, Real &b, Real &c, Real u)
 // reference does (K27:2557-2559).  ft (by direction) is replaced by the post-collision
 // populations.
 template <typename Real>
-__device__ inline void cumulant27(Real *ft, Real rho, Real u, Real v, Real w, Real omegaEff, Real omegaB, Real clim)
+__device__ inline void cumulant27(Real *ft, Real *K, Real rho, Real u, Real v, Real w, Real omegaEff, Real omegaB,
+                                  Real clim)
 {
-    Real K[27];
+    // central moments.  K holds the raw z moments of the nine z lines (zpass_raw27):
+    // shift them to central about w (k1 = k1r - w k0, k2 = k2r - w (k1r + k1), as fwd3),
+    // then the y lines and the x lines; afterwards K[9a+3b+c] = K_abc
 #pragma unroll
-    for (int s = 0; s < 27; s++) K[s] = ft[L27::q_of_slot[s]];
-    // central moments: z lines, y lines, x lines; afterwards K[9a+3b+c] = K_abc
-#pragma unroll
-    for (int p = 0; p < 9; p++) fwd3(K[3 * p], K[3 * p + 1], K[3 * p + 2], w);
+    for (int p = 0; p < 9; p++) {
+        const Real k0 = K[3 * p], k1r = K[3 * p + 1], k2r = K[3 * p + 2];
+        const Real k1 = k1r - w * k0;
+        K[3 * p + 1] = k1;
+        K[3 * p + 2] = k2r - w * (k1r + k1);
+    }
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -427,7 +453,8 @@ __device__ inline void cumulant27(Real *ft, Real rho, Real u, Real v, Real w, Re
     Real X2 = (Real(1) - o1) * (C200 - C002) - Real(3) * rho * (Real(1) - Real(0.5) * o1) * (u * u * Dxu - w * w * Dzw);
     Real X3 = K000 * o2 + (Real(1) - o2) * (C200 + C020 + C002)
               - Real(3) * rho * (Real(1) - Real(0.5) * o2) * (u * u * Dxu + v * v * Dyv + w * w * Dzw);
-    const Real Cs200 = (X1 + X2 + X3) / Real(3), Cs020 = Cs200 - X1, Cs002 = Cs200 - X2;
+    const Real third = Real(1) / Real(3);
+    const Real Cs200 = (X1 + X2 + X3) * third, Cs020 = Cs200 - X1, Cs002 = Cs200 - X2;
     // third order
     X1 = (Real(1) - o3a) * (C120 + C102);
     X2 = (Real(1) - o4a) * (C120 - C102);
@@ -441,13 +468,14 @@ __device__ inline void cumulant27(Real *ft, Real rho, Real u, Real v, Real w, Re
     const Real Cs111 = (Real(1) - o5l) * C111;
     // fourth order with omega6..8 = 1 (only the A/B terms remain); fifth, sixth order -> 0
     const Real vis = rcp(o1) - Real(0.5);
-    X1 = Real(2) * vis * A * rho * (Dxu - Real(2) * Dyv + Dzw) / Real(3);
-    X2 = Real(2) * vis * A * rho * (Dxu + Dyv - Real(2) * Dzw) / Real(3);
-    X3 = Real(-4) * vis * A * rho * (Dxu + Dyv + Dzw) / Real(3);
-    const Real Cs220 = (X1 + X2 + X3) / Real(3), Cs202 = -(X1 - X3) / Real(3), Cs022 = -(X2 - X3) / Real(3);
-    const Real Cs211 = -vis * B * rho * Dyw_Dzv / Real(3);
-    const Real Cs121 = -vis * B * rho * Dxw_Dzu / Real(3);
-    const Real Cs112 = -vis * B * rho * Dxv_Dyu / Real(3);
+    const Real vA = Real(2) * vis * A * rho * third, vB = -vis * B * rho * third;
+    X1 = vA * (Dxu - Real(2) * Dyv + Dzw);
+    X2 = vA * (Dxu + Dyv - Real(2) * Dzw);
+    X3 = Real(-2) * vA * (Dxu + Dyv + Dzw);
+    const Real Cs220 = (X1 + X2 + X3) * third, Cs202 = (X3 - X1) * third, Cs022 = (X3 - X2) * third;
+    const Real Cs211 = vB * Dyw_Dzv;
+    const Real Cs121 = vB * Dxw_Dzu;
+    const Real Cs112 = vB * Dxv_Dyu;
 
     // back to central moments
     const Real Ks200 = Cs200, Ks020 = Cs020, Ks002 = Cs002, Ks110 = Cs110, Ks101 = Cs101, Ks011 = Cs011;
@@ -481,20 +509,27 @@ __device__ inline void cumulant27(Real *ft, Real rho, Real u, Real v, Real w, Re
     KM(2, 2, 1) = Ks221; KM(2, 1, 2) = Ks212; KM(1, 2, 2) = Ks122; KM(2, 2, 2) = Ks222;
 #undef KM
     // populations: x lines, y lines, z lines (K27:2563-2662)
+    const Real hu = Real(0.5) * u, huu = hu * u, hv = Real(0.5) * v, hvv = hv * v, hw = Real(0.5) * w, hww = hw * w;
 #pragma unroll
-    for (int p = 0; p < 9; p++) bwd3(K[p], K[9 + p], K[18 + p], u);
+    for (int p = 0; p < 9; p++) bwd3(K[p], K[9 + p], K[18 + p], u, hu, huu);
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) bwd3(K[9 * a + c], K[9 * a + 3 + c], K[9 * a + 6 + c], v);
+        for (int c = 0; c < 3; c++) bwd3(K[9 * a + c], K[9 * a + 3 + c], K[9 * a + 6 + c], v, hv, hvv);
 #pragma unroll
-    for (int p = 0; p < 9; p++) bwd3(K[3 * p], K[3 * p + 1], K[3 * p + 2], w);
+    for (int p = 0; p < 9; p++) bwd3(K[3 * p], K[3 * p + 1], K[3 * p + 2], w, hw, hww);
 #pragma unroll
     for (int q = 0; q < 27; q++) ft[q] = K[L27::slot[q]];
 }
 
+// At least six waves per SIMD for the 32-bit arithmetic: the cumulant update is VALU bound
+// and fits 64 VGPRs without spilling; the FP64 form keeps the compiler's own choice.
+template <typename Real>
+constexpr int kStep27MinWaves = sizeof(Real) == 8 ? 1 : 6;
+
 template <typename S, typename Real, int SGS>
-__global__ void __launch_bounds__(256) step27_kernel(const StepArgs<Real, 27> a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStep27MinWaves<Real>, 8)))
+step27_kernel(const StepArgs<Real, 27> a)
 {
     const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     const uint32_t icu = blk * blockDim.x + threadIdx.x;
