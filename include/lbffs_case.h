@@ -62,6 +62,13 @@ int lbffs_case_neisolid(int q, int nx, int ny, int nz, const uint8_t *solid, uin
 int lbffs_case_outlet_sponge(int nx, int ny, int nz, int i, int j, int k, int p1, int p2, int p3, int spz,
                              float omega0, float *omega);
 
+/* walls.hpp:372-469 setSDFnearBwalls: sdf = 0.5 + d in the int(dr)+1 layers d = 0, 1, ...
+ * next to every domain face whose face cell has boundary code 1 (BounceBack) in the
+ * face's normal-direction code array (b1 for x faces, b2 for y, b3 for z); a layer that
+ * would reach the opposite face cell is skipped.  sdf: nx*ny*nz floats, updated in place. */
+int lbffs_case_sdf_near_bwalls(float dr, const int32_t *b1, const int32_t *b2, const int32_t *b3, int nx, int ny, int nz,
+                               float *sdf);
+
 #ifdef __cplusplus
 }
 #endif

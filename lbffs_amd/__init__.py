@@ -1,7 +1,7 @@
 """lbffs_amd -- MI355X lattice-Boltzmann time-step engine for the LBFFS solver.
 
-Python view of the C ABI in ``include/lbffs.h`` (device step) and
-``include/lbffs_case.h`` (host case setup).  The library ``lbffs_amd/liblbffs.so`` is
+Python view of the C ABI in ``include/lbffs.h`` (device step), ``include/lbffs_geom.h``
+(geometry preprocessing on the device) and ``include/lbffs_case.h`` (host case setup).  The library ``lbffs_amd/liblbffs.so`` is
 built by ``python -m lbffs_amd.build``; importing this package never builds it and
 never falls back to another implementation: without the library every call raises.
 """
@@ -113,13 +113,17 @@ def lib():
         L.lbffs_batch_step.argtypes = [vp, ip, C.POINTER(StepParams)]
         L.lbffs_batch_last_step_ms.argtypes = [vp, C.POINTER(C.c_float)]
         L.lbffs_batch_destroy.argtypes = [vp]
+        fl = C.c_float
+        L.lbffs_geom_solid.argtypes = [ip, vp, vp, vp, ip, ip, ip, ip, vp]
+        L.lbffs_geom_sdf.argtypes = [ip, vp, vp, fl, fl, fl, ip, ip, ip, vp, vp]
+        L.lbffs_case_sdf_near_bwalls.argtypes = [fl, vp, vp, vp, ip, ip, ip, vp]
         for n in ("lbffs_create", "lbffs_upload", "lbffs_download", "lbffs_step", "lbffs_reduce",
                   "lbffs_synchronize", "lbffs_last_step_ms", "lbffs_destroy", "lbffs_case_units",
                   "lbffs_case_faces", "lbffs_case_init_feq", "lbffs_case_neisolid", "lbffs_case_outlet_sponge",
                   "lbffs_probe_copy", "lbffs_probe_bandwidth", "lbffs_autotune", "lbffs_set_launch",
                   "lbffs_ibm_set_markers", "lbffs_ibm_set_motion", "lbffs_ibm_download", "lbffs_batch_create",
                   "lbffs_batch_context", "lbffs_batch_set_launch", "lbffs_batch_step", "lbffs_batch_last_step_ms",
-                  "lbffs_batch_destroy"):
+                  "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf", "lbffs_case_sdf_near_bwalls"):
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib

@@ -3,6 +3,8 @@
 // reductions.  Replaces the OpenCL host section of D3Q19_opencl/ns.cpp:382-603.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -10,10 +12,12 @@
 #include <vector>
 
 #include "../../include/lbffs.h"
+#include "../../include/lbffs_geom.h"
 #include "d3q19_step.hpp"
 #include "d3q27_step.hpp"
 #include "ibm.hpp"
 #include "batch.hpp"
+#include "geom.hpp"
 
 using namespace lbffs;
 
@@ -1239,6 +1243,168 @@ int lbffs_batch_destroy(lbffs_batch *b)
     if (!b) return 0;
     batch_free(b);
     delete b;
+    return 0;
+}
+
+}  // extern "C"
+
+// ---- geometry preprocessing (include/lbffs_geom.h) ----
+
+namespace {
+
+// device buffers of one geometry call, freed on every return path
+struct GeomScratch {
+    std::vector<void *> bufs;
+    template <typename T>
+    int alloc(T **p, size_t count)
+    {
+        void *q = nullptr;
+        int e = dalloc(&q, count * sizeof(T) > 0 ? count * sizeof(T) : 1);
+        if (e) return e;
+        bufs.push_back(q);
+        *p = (T *)q;
+        return 0;
+    }
+    ~GeomScratch()
+    {
+        for (void *q : bufs) hipFree(q);
+    }
+};
+
+int geom_dims_ok(int nx, int ny, int nz)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return fail(LBFFS_E_INVALID, "lattice dimensions must be positive");
+    if ((double)nx * ny * nz >= 4294967295.0) return fail(LBFFS_E_INVALID, "lattice too large (2^32 cells or more)");
+    return 0;
+}
+
+unsigned geom_grid(size_t n)
+{
+    return (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbffs_geom_solid(int n, const float *v0, const float *v1, const float *v2, int nx, int ny, int nz, int invert,
+                     uint8_t *solid)
+{
+    if (!solid || (n > 0 && (!v0 || !v1 || !v2))) return fail(LBFFS_E_INVALID, "null argument");
+    if (n < 0) return fail(LBFFS_E_INVALID, "negative triangle count");
+    if (int e = geom_dims_ok(nx, ny, nz)) return e;
+    const size_t N = (size_t)nx * ny * nz;
+    // bounding box of the surface, grown from the origin (stl.hpp:525-550), and of each
+    // triangle (stl.hpp:224-262); min/max are exact, so the order of the reduction is free
+    lbffs::GeomBox box = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    std::vector<lbffs::GeomTri> tris((size_t)n);
+    for (int t = 0; t < n; t++) {
+        lbffs::GeomTri &g = tris[t];
+        for (int a = 0; a < 3; a++) {
+            g.v0[a] = v0[(size_t)a * n + t];
+            g.v1[a] = v1[(size_t)a * n + t];
+            g.v2[a] = v2[(size_t)a * n + t];
+        }
+        g.ymin = fminf(g.v0[1], fminf(g.v1[1], g.v2[1]));
+        g.ymax = fmaxf(g.v0[1], fmaxf(g.v1[1], g.v2[1]));
+        g.zmin = fminf(g.v0[2], fminf(g.v1[2], g.v2[2]));
+        g.zmax = fmaxf(g.v0[2], fmaxf(g.v1[2], g.v2[2]));
+        g.pad = 0.f;
+        box.xmin = fminf(box.xmin, fminf(g.v0[0], fminf(g.v1[0], g.v2[0])));
+        box.xmax = fmaxf(box.xmax, fmaxf(g.v0[0], fmaxf(g.v1[0], g.v2[0])));
+        box.ymin = fminf(box.ymin, g.ymin);
+        box.ymax = fmaxf(box.ymax, g.ymax);
+        box.zmin = fminf(box.zmin, g.zmin);
+        box.zmax = fmaxf(box.zmax, g.zmax);
+    }
+    // triangles by the z layers their box may cover (a superset: the kernel applies the
+    // exact test of walls.hpp:116-121); a box with a NaN bound covers no layer, as the
+    // reference's comparisons fail for it
+    std::vector<int> bin_start((size_t)nz + 1, 0), bin_tri;
+    {
+        std::vector<std::pair<int, int>> span((size_t)n, {1, 0});
+        for (int t = 0; t < n; t++) {
+            const float lo = tris[t].zmin, hi = tris[t].zmax;
+            if (!(lo <= hi)) continue;
+            const double klo = std::max(std::floor((double)lo) - 1.0, 0.0);
+            const double khi = std::min(std::ceil((double)hi) + 1.0, (double)(nz - 1));
+            if (klo > khi) continue;
+            span[t] = {(int)klo, (int)khi};
+            for (int k = (int)klo; k <= (int)khi; k++) bin_start[k + 1]++;
+        }
+        for (int k = 0; k < nz; k++) bin_start[k + 1] += bin_start[k];
+        bin_tri.resize((size_t)bin_start[nz]);
+        std::vector<int> fill(bin_start.begin(), bin_start.end() - 1);
+        for (int t = 0; t < n; t++)
+            for (int k = span[t].first; k <= span[t].second; k++) bin_tri[(size_t)fill[k]++] = t;
+    }
+    GeomScratch g;
+    uint8_t *d_solid;
+    lbffs::GeomTri *d_tris;
+    int *d_start, *d_bins;
+    if (int e = g.alloc(&d_solid, N)) return e;
+    if (int e = g.alloc(&d_tris, tris.size())) return e;
+    if (int e = g.alloc(&d_start, bin_start.size())) return e;
+    if (int e = g.alloc(&d_bins, bin_tri.size())) return e;
+    HIP_TRY(hipMemcpy(d_solid, solid, N, hipMemcpyHostToDevice));
+    if (n) HIP_TRY(hipMemcpy(d_tris, tris.data(), tris.size() * sizeof(lbffs::GeomTri), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_start, bin_start.data(), bin_start.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!bin_tri.empty()) HIP_TRY(hipMemcpy(d_bins, bin_tri.data(), bin_tri.size() * sizeof(int), hipMemcpyHostToDevice));
+    const int chunks = (nx + lbffs::kGeomRow - 1) / lbffs::kGeomRow;
+    const size_t blocks = (size_t)chunks * ny * nz;
+    if (blocks > 0x7fffffffu) return fail(LBFFS_E_INVALID, "lattice too large for the solid launch");
+    hipLaunchKernelGGL(lbffs::geom_solid_kernel, dim3((unsigned)blocks), dim3(lbffs::kGeomRow), 0, nullptr, d_tris,
+                       d_start, d_bins, nx, ny, chunks, box, d_solid);
+    HIP_TRY(hipGetLastError());
+    if (invert) {
+        hipLaunchKernelGGL(lbffs::geom_invert_kernel, dim3(geom_grid(N)), dim3(256), 0, nullptr, d_solid, N);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpy(solid, d_solid, N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int lbffs_geom_sdf(int n, const float *centroid, const float *normal, float sdf_ini, float dr, float p, int nx, int ny,
+                   int nz, const uint8_t *neiSolid, float *sdf)
+{
+    if (!neiSolid || !sdf || (n > 0 && (!centroid || !normal))) return fail(LBFFS_E_INVALID, "null argument");
+    if (n < 0) return fail(LBFFS_E_INVALID, "negative triangle count");
+    if (int e = geom_dims_ok(nx, ny, nz)) return e;
+    if (!(dr >= 0.f) || dr > 1e6f) return fail(LBFFS_E_INVALID, "dr must be in [0, 1e6]");
+    const size_t N = (size_t)nx * ny * nz;
+    const int drn = (int)dr;
+    GeomScratch g;
+    uint8_t *d_nei, *d_m1, *d_m2;
+    uint32_t *d_list, *d_count;
+    float *d_tri, *d_sdf;
+    if (int e = g.alloc(&d_nei, N)) return e;
+    if (int e = g.alloc(&d_m1, N)) return e;
+    if (int e = g.alloc(&d_m2, N)) return e;
+    if (int e = g.alloc(&d_list, N)) return e;
+    if (int e = g.alloc(&d_count, 1)) return e;
+    if (int e = g.alloc(&d_tri, (size_t)6 * n)) return e;
+    if (int e = g.alloc(&d_sdf, N)) return e;
+    HIP_TRY(hipMemcpy(d_nei, neiSolid, N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_sdf, sdf, N * sizeof(float), hipMemcpyHostToDevice));
+    if (n) {
+        HIP_TRY(hipMemcpy(d_tri, centroid, (size_t)3 * n * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_tri + (size_t)3 * n, normal, (size_t)3 * n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    const unsigned gr = geom_grid(N);
+    hipLaunchKernelGGL(lbffs::geom_dilate_kernel, dim3(gr), dim3(256), 0, nullptr, d_nei, d_m1, nx, ny, nz, 0, drn, 1);
+    hipLaunchKernelGGL(lbffs::geom_dilate_kernel, dim3(gr), dim3(256), 0, nullptr, d_m1, d_m2, nx, ny, nz, 1, drn, 0);
+    hipLaunchKernelGGL(lbffs::geom_dilate_kernel, dim3(gr), dim3(256), 0, nullptr, d_m2, d_m1, nx, ny, nz, 2, drn, 0);
+    hipLaunchKernelGGL(lbffs::geom_near_list_kernel, dim3(gr), dim3(256), 0, nullptr, d_m1, N, d_list, d_count);
+    HIP_TRY(hipGetLastError());
+    uint32_t count = 0;
+    HIP_TRY(hipMemcpy(&count, d_count, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (count > N) return fail(LBFFS_E_DEVICE, "near-cell count exceeds the lattice");
+    if (count) {
+        hipLaunchKernelGGL(lbffs::geom_sdf_kernel, dim3((count + 255) / 256), dim3(256), 0, nullptr, d_list, count, d_tri,
+                           n, nx, ny, sdf_ini, dr, p, d_sdf);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpy(sdf, d_sdf, N * sizeof(float), hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -210,4 +210,41 @@ int lbffs_case_outlet_sponge(int nx, int ny, int nz, int i, int j, int k, int p1
     return 0;
 }
 
+int lbffs_case_sdf_near_bwalls(float dr, const int32_t *b1, const int32_t *b2, const int32_t *b3, int nx, int ny, int nz,
+                               float *sdf)
+{
+    if (!b1 || !b2 || !b3 || !sdf || nx < 1 || ny < 1 || nz < 1) return -1;
+    const int n = int(dr) + 1;  // layers 0 .. int(dr) from each bounce-back face
+    // x faces: the code on the face cell decides; the last layer before the opposite
+    // face is left alone (i < nx-1, nx-1-i > 0)
+    for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++) {
+            if (b1[idx(0, j, k, nx, ny)] == 1)
+                for (int i = 0; i < n; i++)
+                    if (i < nx - 1) sdf[idx(i, j, k, nx, ny)] = 0.5f + i;
+            if (b1[idx(nx - 1, j, k, nx, ny)] == 1)
+                for (int i = 0; i < n; i++)
+                    if (nx - 1 - i > 0) sdf[idx(nx - 1 - i, j, k, nx, ny)] = 0.5f + i;
+        }
+    for (int i = 0; i < nx; i++)
+        for (int k = 0; k < nz; k++) {
+            if (b2[idx(i, 0, k, nx, ny)] == 1)
+                for (int j = 0; j < n; j++)
+                    if (j < ny - 1) sdf[idx(i, j, k, nx, ny)] = 0.5f + j;
+            if (b2[idx(i, ny - 1, k, nx, ny)] == 1)
+                for (int j = 0; j < n; j++)
+                    if (ny - 1 - j > 0) sdf[idx(i, ny - 1 - j, k, nx, ny)] = 0.5f + j;
+        }
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            if (b3[idx(i, j, 0, nx, ny)] == 1)
+                for (int k = 0; k < n; k++)
+                    if (k < nz - 1) sdf[idx(i, j, k, nx, ny)] = 0.5f + k;
+            if (b3[idx(i, j, nz - 1, nx, ny)] == 1)
+                for (int k = 0; k < n; k++)
+                    if (nz - 1 - k > 0) sdf[idx(i, j, nz - 1 - k, nx, ny)] = 0.5f + k;
+        }
+    return 0;
+}
+
 }  // extern "C"

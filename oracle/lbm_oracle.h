@@ -94,6 +94,13 @@ void orc_ibm_spread_move_f64(const double *m, int n, double *xyz, int nx, int ny
 void orc_ibm_force_f64(int Q, long N, double *f, const unsigned char *solid, const double *rho, const double *Gx,
                        const double *Gy, const double *Gz);
 
+/* geometry preprocessing (geom_oracle.c): walls.hpp:18-184 setSolid, walls.hpp:218-297 setSDF */
+int orc_geom_is_cross(const float *v0, const float *v1, const float *v2, int i, int j, int k);
+int orc_geom_solid(int n, const float *v0, const float *v1, const float *v2, int nx, int ny, int nz, int invert,
+                   unsigned char *solid);
+int orc_geom_sdf(int n, const float *c, const float *nrm, float sdfIni, float dr, float p, int nx, int ny, int nz,
+                 const unsigned char *neiSolid, float *sdf);
+
 #ifdef __cplusplus
 }
 #endif

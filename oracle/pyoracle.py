@@ -80,7 +80,30 @@ def _bind(L):
         getattr(L, f"orc_ibm_gwall_{p}").argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int] + [vp] * 6
         getattr(L, f"orc_ibm_spread_move_{p}").argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int] + [vp] * 6
         getattr(L, f"orc_ibm_force_{p}").argtypes = [C.c_int, C.c_long] + [vp] * 6
+    L.orc_geom_is_cross.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int]
+    L.orc_geom_solid.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+    L.orc_geom_sdf.argtypes = [C.c_int, vp, vp, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, vp, vp]
     return L
+
+
+def geom_solid(v0, v1, v2, nx, ny, nz, invert=False, solid=None):
+    """Restated setSolid (walls.hpp:159-184): v0, v1, v2 are (3, n) float32 rows."""
+    v = [np.ascontiguousarray(x, np.float32) for x in (v0, v1, v2)]
+    s = np.zeros(nx * ny * nz, np.uint8) if solid is None else np.ascontiguousarray(solid, np.uint8).copy()
+    rc = orc().orc_geom_solid(v[0].shape[1], *[_p(x) for x in v], nx, ny, nz, int(bool(invert)), _p(s))
+    assert rc == 0
+    return s
+
+
+def geom_sdf(c, nrm, sdf, nei, nx, ny, nz, dr=10.0, p=7.0, sdf_ini=10000.0):
+    """Restated setSDF, patch form (walls.hpp:218-297); returns an updated copy of sdf."""
+    c = np.ascontiguousarray(c, np.float32)
+    nrm = np.ascontiguousarray(nrm, np.float32)
+    out = np.ascontiguousarray(sdf, np.float32).copy()
+    nei = np.ascontiguousarray(nei, np.uint8)
+    rc = orc().orc_geom_sdf(c.shape[1], _p(c), _p(nrm), sdf_ini, dr, p, nx, ny, nz, _p(nei), _p(out))
+    assert rc == 0
+    return out
 
 
 def _p(a):

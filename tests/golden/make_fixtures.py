@@ -7,8 +7,11 @@ Writes, next to this script:
   ghia_re100_centerline.csv   y, u/U of Ghia, Ghia & Shin (1982), Re=100 (cavityFlow/Ghia.dat)
   cavity_re100_u_x05.csv      y, u of the reference's own 100x100 cavity run (cavityFlow/u_x05.csv)
   poiseuille_u.csv            y, u of the reference's own 100x20 channel run (poiseuilleFlow/u.csv)
+  cylinder.stl.gz             walls STL of flowAroundCylinder (one patch, 3664 triangles), gzip'ed
+  backStepBoxes.stl.gz        walls STL of backStepFlow (patches Top, Bottom), gzip'ed
 """
 import csv
+import gzip
 import os
 import sys
 
@@ -40,4 +43,8 @@ write("cavity_re100_u_x05.csv", ["y", "u"],
       [[p, u] for u, p in read_csv_cols(os.path.join(ref, "testCases/cavityFlow/u_x05.csv"), ["u", "Points:1"])])
 write("poiseuille_u.csv", ["y", "u"],
       [[p, u] for u, p in read_csv_cols(os.path.join(ref, "testCases/poiseuilleFlow/u.csv"), ["u", "Points:1"])])
+for src, name in (("testCases/flowAroundCylinder/cylinder.stl", "cylinder.stl.gz"),
+                  ("testCases/backStepFlow/backStepBoxes.stl", "backStepBoxes.stl.gz")):
+    with open(os.path.join(ref, src), "rb") as fh, gzip.GzipFile(os.path.join(here, name), "wb", 9, mtime=0) as out:
+        out.write(fh.read())
 print("fixtures written to", here)

@@ -13,7 +13,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared():
     names = set()
-    for h in ("lbffs.h", "lbffs_case.h"):
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         names |= set(re.findall(r"\b(?:int|const char \*)\s*(lbffs_\w+)\s*\(", text))
@@ -58,6 +60,12 @@ def test_strerror_and_invalid_arguments():
     assert L.lbffs_batch_step(None, 1, None) == -1
     assert L.lbffs_batch_destroy(None) == 0
     assert L.lbffs_upload(None, 0, None, 0) == -1
+    # geometry: argument checks come before any device call
+    assert L.lbffs_geom_solid(1, None, None, None, 4, 4, 4, 0, None) == -1
+    assert L.lbffs_geom_sdf(1, None, None, 1.0, 1.0, 7.0, 4, 4, 4, None, None) == -1
+    buf = (ctypes.c_uint8 * 8)()
+    assert L.lbffs_geom_solid(0, None, None, None, 0, 2, 2, 0, buf) == -1
+    assert L.lbffs_geom_solid(-1, None, None, None, 2, 2, 2, 0, buf) == -1
     assert L.lbffs_step(None, 1, None) == -1
     assert L.lbffs_destroy(None) == 0
 

@@ -1,0 +1,346 @@
+"""Geometry preprocessing on the CPU: the STL reader of the host mirror, the oracle's
+setSolid / setSDF pinned to exact arithmetic, closed forms and analytic shapes, and the
+host setSDFnearBwalls.
+
+The oracle's solid test is pinned three ways:
+  - meshes with small integer vertices, where every FP32 operation of the crossing test
+    is exact up to the final divisions (whose rounding cannot move a quotient across 0
+    or 1), so the reference's decisions -- including the 0/0 = NaN case of a ray lying in
+    a triangle's plane, and edges shared by two triangles -- follow from exact rational
+    arithmetic (fractions.Fraction);
+  - the reference's own cylinder STL (flowAroundCylinder) against the analytic cylinder
+    away from its surface;
+  - the reference's backward-step boxes (backStepFlow) against the analytic boxes.
+"""
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import lbffs_amd as lb
+from lbffs_amd import cases
+from lbffs_amd import geometry as geo
+
+import pyoracle as po
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CYL_STL = os.path.join(GOLD, "cylinder.stl.gz")
+STEP_STL = os.path.join(GOLD, "backStepBoxes.stl.gz")
+L_CYL = float(np.float32(6.0) / np.float32(1199.0))    # flowAroundCylinder: Lx 6, nx 1200
+L_STEP = float(np.float32(50.0) / np.float32(1499.0))  # backStepFlow: Lx 50, nx 1500
+
+
+# ---- the STL reader -------------------------------------------------------------------
+
+def test_tokens_follow_boost_split():
+    assert geo.tokens("solid A\n") == ["solid", "A"]
+    # an indented line starts with an empty token, trailing blanks end with one
+    assert geo.tokens("  facet normal 0 0 1 \n") == ["", "facet", "normal", "0", "0", "1", ""]
+    assert geo.tokens("a\r\n\n b") == ["a", "", "", "", "b"]
+
+
+def _facet(n, a, b, c):
+    f = lambda v: " ".join(str(x) for x in v)
+    return (f"  facet normal {f(n)}\n    outer loop\n      vertex {f(a)}\n      vertex {f(b)}\n"
+            f"      vertex {f(c)}\n    endloop\n  endfacet\n")
+
+
+def test_parse_patches_scaling_and_centroids():
+    text = ("solid first\n" + _facet((0, 0, 1), (0, 0, 0), (3, 0, 0), (0, 3, 0)) + "endsolid first\n"
+            "solid second\n" + _facet((1, 0, 0), (1.5, 3, 6), (1.5, 0, 0), (1.5, 6, 0))
+            + _facet((1, 0, 0), (0, 0, 0), (0, 0, 0), (0, 0, 0)) + "endsolid second\n")
+    s = geo.parse_stl(text, 0.5)
+    assert [p.name for p in s.patches] == ["first", "second"]
+    assert [p.n for p in s.patches] == [1, 2]
+    p0, p1 = s.patches
+    # vertices divided by L, normals untouched, centroids (v0 + v1 + v2) / 3
+    assert p0.v1[:, 0].tolist() == [6.0, 0.0, 0.0]
+    assert p0.c[:, 0].tolist() == [2.0, 2.0, 0.0]
+    assert p1.normal[:, 0].tolist() == [1.0, 0.0, 0.0]
+    assert p1.v0[:, 0].tolist() == [3.0, 6.0, 12.0]
+    v0, v1, v2 = s.triangles()
+    assert v0.shape == (3, 3) and v0.dtype == np.float32
+
+
+def test_parse_repeated_patch_name_joins_first_patch():
+    # the reader moves to the next patch only when the name differs (stl.hpp:478-485)
+    text = ("solid A\n" + _facet((0, 0, 1), (0, 0, 0), (1, 0, 0), (0, 1, 0)) + "endsolid A\n"
+            "solid A\n" + _facet((0, 0, 1), (0, 0, 1), (1, 0, 1), (0, 1, 1)) + "endsolid A\n")
+    s = geo.parse_stl(text, 1.0)
+    assert [p.n for p in s.patches] == [2, 0]
+
+
+def test_stof_is_strtof():
+    # std::stof parses the decimal string straight to the nearest float
+    assert geo._stof("0.1") == np.float32(0.1)
+    assert geo._stof("1e-50") == np.float32(0.0)
+    assert geo._stof("3.4028236e38") == np.float32(np.inf)
+    with pytest.raises(ValueError):
+        geo._stof("abc")
+
+
+def test_missing_file_reads_empty(tmp_path):
+    assert geo.read_stl(str(tmp_path / "none.stl"), 1.0).patches == []
+
+
+def test_cylinder_fixture():
+    s = geo.read_stl(CYL_STL, L_CYL)
+    assert [p.name for p in s.patches] == ["Mesh"] and s.patches[0].n == 3664
+    v0, v1, v2 = s.triangles()
+    v = np.concatenate([v0, v1, v2], axis=1).astype(np.float64) * L_CYL
+    r = np.hypot(v[0] - 1.0, v[1] - 1.0)
+    assert r.max() <= 0.05 + 1e-5 and abs(v[2]).max() <= 0.05 + 1e-6
+
+
+# ---- exact model of the crossing test -------------------------------------------------
+
+def _det(a, b, c):
+    return (a[0] * b[1] * c[2] + a[1] * b[2] * c[0] + a[2] * b[0] * c[1]
+            - a[0] * b[2] * c[1] - a[1] * b[0] * c[2] - a[2] * b[1] * c[0])
+
+
+def exact_is_cross(v0, v1, v2, i, j, k):
+    """True/False, or None where only the sign of a zero would decide (z = +-inf)."""
+    F = lambda x: Fraction(float(x)) if not isinstance(x, Fraction) else x
+    a = [F(v1[t]) - F(v0[t]) for t in range(3)]
+    b = [F(v2[t]) - F(v0[t]) for t in range(3)]
+    c = [F(-1), F(0), F(0)]
+    d = [F(i) - F(v0[0]), F(j) - F(v0[1]), F(k) - F(v0[2])]
+    den, nxx, nyy, nzz = _det(a, b, c), _det(d, b, c), _det(a, d, c), _det(a, b, d)
+    if den == 0:
+        # x or y infinite fails a range test; x, y, z all 0/0 = NaN pass every test
+        if nxx != 0 or nyy != 0:
+            return False
+        return True if nzz == 0 else None
+    x, y, z = nxx / den, nyy / den, nzz / den
+    return 0 <= x <= 1 and 0 <= y <= 1 and 0 <= x + y <= 1 and z >= 0
+
+
+def exact_solid(v0, v1, v2, nx, ny, nz):
+    """setSolid by exact arithmetic; -1 marks an undetermined cell."""
+    n = v0.shape[1]
+    V = [[[Fraction(float(v[a, t])) for a in range(3)] for t in range(n)] for v in (v0, v1, v2)]
+    lo = [min([Fraction(0)] + [V[s][t][a] for s in range(3) for t in range(n)]) for a in range(3)]
+    hi = [max([Fraction(0)] + [V[s][t][a] for s in range(3) for t in range(n)]) for a in range(3)]
+    out = np.zeros(nx * ny * nz, np.int8)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                if not all(lo[a] <= x <= hi[a] for a, x in enumerate((i, j, k))):
+                    continue
+                cnt, unknown = 0, False
+                for t in range(n):
+                    ys = [V[s][t][1] for s in range(3)]
+                    zs = [V[s][t][2] for s in range(3)]
+                    if not (min(ys) <= j <= max(ys) and min(zs) <= k <= max(zs)):
+                        continue
+                    r = exact_is_cross(V[0][t], V[1][t], V[2][t], i, j, k)
+                    if r is None:
+                        unknown = True
+                    cnt += bool(r)
+                out[i + nx * (j + ny * k)] = -1 if unknown else (cnt % 2)
+    return out
+
+
+def _mesh(tris):
+    t = np.array(tris, np.float32)  # (n, 3 vertices, 3 axes)
+    return [np.ascontiguousarray(t[:, s, :].T) for s in range(3)]
+
+
+def box_mesh(lo, hi):
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    P = lambda a, b, c: (x1 if a else x0, y1 if b else y0, z1 if c else z0)
+    quads = [(P(0, 0, 0), P(0, 1, 0), P(0, 1, 1), P(0, 0, 1)), (P(1, 0, 0), P(1, 0, 1), P(1, 1, 1), P(1, 1, 0)),
+             (P(0, 0, 0), P(0, 0, 1), P(1, 0, 1), P(1, 0, 0)), (P(0, 1, 0), P(1, 1, 0), P(1, 1, 1), P(0, 1, 1)),
+             (P(0, 0, 0), P(1, 0, 0), P(1, 1, 0), P(0, 1, 0)), (P(0, 0, 1), P(0, 1, 1), P(1, 1, 1), P(1, 0, 1))]
+    tris = []
+    for a, b, c, d in quads:
+        tris += [(a, b, c), (a, c, d)]
+    return _mesh(tris)
+
+
+def octa_mesh(c, r):
+    cx, cy, cz = c
+    X = [(cx + r, cy, cz), (cx - r, cy, cz)]
+    Y = [(cx, cy + r, cz), (cx, cy - r, cz)]
+    Z = [(cx, cy, cz + r), (cx, cy, cz - r)]
+    return _mesh([(x, y, z) for x in X for y in Y for z in Z])
+
+
+def tetra_mesh():
+    p = [(1, 1, 1), (8, 2, 1), (3, 7, 2), (4, 3, 6)]
+    return _mesh([(p[0], p[1], p[2]), (p[0], p[1], p[3]), (p[0], p[2], p[3]), (p[1], p[2], p[3])])
+
+
+ALIGNED = {
+    "box": (lambda: box_mesh((2, 1, 0), (7, 5, 3)), (10, 7, 5)),
+    "octahedron": (lambda: octa_mesh((5, 4, 3), 3), (11, 9, 7)),
+    "tetrahedron": (tetra_mesh, (10, 9, 8)),
+    "two_boxes": (lambda: [np.concatenate(p, axis=1) for p in zip(box_mesh((1, 1, 1), (4, 4, 4)),
+                                                                   box_mesh((4, 1, 1), (7, 3, 3)))], (9, 6, 6)),
+}
+
+
+@pytest.mark.parametrize("name", list(ALIGNED))
+def test_oracle_solid_matches_exact_arithmetic(name):
+    make, (nx, ny, nz) = ALIGNED[name]
+    v0, v1, v2 = make()
+    ex = exact_solid(v0, v1, v2, nx, ny, nz)
+    orc = po.geom_solid(v0, v1, v2, nx, ny, nz)
+    known = ex >= 0
+    assert known.mean() > 0.9
+    assert np.array_equal(orc[known], ex[known].astype(np.uint8)), name
+    assert orc.sum() > 0
+
+
+def test_exact_model_sees_the_degenerate_cases():
+    # the box's faces lie on lattice planes: rays in the planes y = 1, y = 5, z = 0, z = 3
+    # meet triangles edge-on (0/0), and cells on x = 2 sit on a face
+    v0, v1, v2 = box_mesh((2, 1, 0), (7, 5, 3))
+    nan_hits = sum(exact_is_cross(v0[:, t], v1[:, t], v2[:, t], 0, 1, 1) is True for t in range(12))
+    assert nan_hits > 0
+    ex = exact_solid(v0, v1, v2, 10, 7, 5)
+    interior = [(i, j, k) for i in range(3, 7) for j in range(2, 5) for k in range(1, 3)]
+    assert all(ex[i + 10 * (j + 7 * k)] == 1 for i, j, k in interior)
+
+
+def test_oracle_solid_invert_and_keep():
+    v0, v1, v2 = box_mesh((2, 1, 0), (7, 5, 3))
+    rng = np.random.default_rng(3)
+    init = rng.integers(0, 3, 10 * 7 * 5).astype(np.uint8)
+    kept = po.geom_solid(v0, v1, v2, 10, 7, 5, solid=init)
+    plain = po.geom_solid(v0, v1, v2, 10, 7, 5)
+    assert np.array_equal(kept, np.where(plain == 1, 1, init))
+    inv = po.geom_solid(v0, v1, v2, 10, 7, 5, invert=True, solid=init)
+    assert np.array_equal(inv, np.where(kept == 1, 0, 1))
+
+
+def test_oracle_solid_cylinder_analytic():
+    s = geo.read_stl(CYL_STL, L_CYL)
+    v0, v1, v2 = s.triangles()
+    nx, ny, nz = 240, 240, 12
+    sol = po.geom_solid(v0, v1, v2, nx, ny, nz).reshape(nz, ny, nx)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    rho = np.hypot(i - 1.0 / L_CYL, j - 1.0 / L_CYL)
+    # the mesh's side vertices lie between radii 0.04848 and 0.05 m (a faceted, uneven
+    # surface), its caps at z = +-0.05 m
+    R_in, R_out, H = 0.04848 / L_CYL, 0.05 / L_CYL, 0.05 / L_CYL
+    inside = (rho < R_in - 0.05) & (k < H - 0.05)
+    outside = (rho > R_out + 0.05) | (k > H + 0.05)
+    assert sol[inside].all() and not sol[outside].any()
+    assert inside.sum() > 2000
+
+
+def test_oracle_solid_backstep_analytic():
+    # backStepBoxes.stl: patch Bottom is a slab y < 0.99 m for x < 10 m, open at the domain
+    # walls and closed by its face x = 10 m, with ribs up to y = 1.14 m near x = 1.1-1.55 m;
+    # patch Top holds ribs hanging from y = 3 m down to 2.85 m at the same x
+    s = geo.read_stl(STEP_STL, L_STEP)
+    v0, v1, v2 = s.triangles()
+    nx, ny, nz = 360, 91, 20
+    sol = po.geom_solid(v0, v1, v2, nx, ny, nz).reshape(nz, ny, nx)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x, y, z = i * L_STEP, j * L_STEP, k * L_STEP
+    m = 0.01  # metres, a third of a cell
+    slab = (x > m) & (x < 10 - m) & (y > m) & (y < 0.99 - m) & (z > m)
+    gap = (y > 1.14 + m) & (y < 2.85 - m)
+    beyond = (x > 10 + m) & (y < 2.85 - m)
+    assert slab.sum() > 100000 and gap.sum() > 100000
+    assert sol[slab].all()
+    assert not sol[gap | beyond].any()
+
+
+# ---- the signed distance --------------------------------------------------------------
+
+def _plane_patch(z0, n_side=12, spacing=1.0, offset=(2.3, 1.7)):
+    """Triangles of the plane z = z0 with normal +z (two per square)."""
+    tris = []
+    for a in range(n_side):
+        for b in range(n_side):
+            x, y = offset[0] + a * spacing, offset[1] + b * spacing
+            p = [(x, y, z0), (x + spacing, y, z0), (x + spacing, y + spacing, z0), (x, y + spacing, z0)]
+            tris += [(p[0], p[1], p[2]), (p[0], p[2], p[3])]
+    v0, v1, v2 = _mesh(tris)
+    c = ((v0 + v1) + v2) / np.float32(3.0)
+    nrm = np.zeros_like(c)
+    nrm[2] = 1.0
+    return c, nrm
+
+
+def test_oracle_sdf_plane_is_the_distance():
+    nx, ny, nz = 16, 16, 12
+    z0 = 5.25
+    c, nrm = _plane_patch(z0)
+    nei = np.zeros(nx * ny * nz, np.uint8)
+    nei[8 + nx * (8 + ny * 5)] = 1
+    sdf = np.full(nx * ny * nz, 10000.0, np.float32)
+    out = po.geom_sdf(c, nrm, sdf, nei, nx, ny, nz, dr=3.0).reshape(nz, ny, nx)
+    k = np.arange(nz)[:, None, None] * np.ones((1, ny, nx))
+    box = np.zeros((nz, ny, nx), bool)
+    box[2:9, 5:12, 5:12] = True  # int(dr) = 3 around (8, 8, 5)
+    changed = out != 10000.0
+    assert not changed[~box].any()
+    # the layers k = 3..7 (|k - z0| <= 2.25) have centroids within dr; k = 2 has none
+    inner = box.copy()
+    inner[[2, 8]] = False
+    assert changed[inner].all() and not changed[2].any()
+    # the weighted mean of equal plane distances is that distance
+    assert np.allclose(out[changed], (k - z0)[changed], rtol=0, atol=2e-6)
+
+
+def test_oracle_sdf_weighted_mean_closed_form():
+    # two triangles, centroids at distances r1, r2 of a cell; sd from the normals
+    c = np.array([[3.0, 5.5], [2.0, 2.0], [2.0, 2.0]], np.float32)
+    nrm = np.array([[1.0, 0.6], [0.0, 0.0], [0.0, 0.8]], np.float32)
+    nx, ny, nz = 8, 5, 5
+    nei = np.zeros(nx * ny * nz, np.uint8)
+    nei[4 + nx * (2 + ny * 2)] = 1
+    out = po.geom_sdf(c, nrm, np.full(nx * ny * nz, 7.0, np.float32), nei, nx, ny, nz, dr=2.0, p=7.0)
+    ic = 4 + nx * (2 + ny * 2)  # cell (4, 2, 2)
+    d1 = np.array([1.0, 0.0, 0.0]); d2 = np.array([-1.5, 0.0, 0.0])
+    w1, w2 = 1.0 ** -7, 1.5 ** -7
+    sd1, sd2 = d1 @ [1.0, 0.0, 0.0], d2 @ [0.6, 0.0, 0.8]
+    assert abs(out[ic] - (w1 * sd1 + w2 * sd2) / (w1 + w2)) < 1e-6
+    # a near cell with no centroid within dr keeps sdfIni; cells outside the box keep theirs
+    far = 6 + nx * (4 + ny * 4)
+    assert out[far] == np.float32(10000.0)
+    assert out[0] == np.float32(7.0)
+
+
+# ---- setSDFnearBwalls (host) ----------------------------------------------------------
+
+def test_sdf_near_bwalls_layers():
+    nx, ny, nz = 9, 7, 6
+    N = nx * ny * nz
+    b1 = np.zeros(N, np.int32)
+    b2 = np.zeros(N, np.int32)
+    b3 = np.zeros(N, np.int32)
+    idx = lambda i, j, k: i + nx * (j + ny * k)
+    for j in range(ny):
+        for k in range(nz):
+            b1[idx(0, j, k)] = 1          # BounceBack on nxMin only
+    for i in range(nx):
+        for j in range(ny):
+            b3[idx(i, j, nz - 1)] = 1     # and on nzMax
+    sdf = np.full(N, 10000.0, np.float32)
+    geo.sdf_near_bwalls(sdf, b1, b2, b3, nx, ny, nz, dr=3.0)
+    s = sdf.reshape(nz, ny, nx)
+    want = np.full((nz, ny, nx), 10000.0, np.float32)
+    for d in range(4):
+        want[:, :, d] = 0.5 + d                  # x layers 0..3
+    for d in range(4):
+        want[nz - 1 - d, :, :] = 0.5 + d         # z layers, written after x
+    assert np.array_equal(s, want)
+
+
+def test_sdf_near_bwalls_skips_the_opposite_face():
+    nx, ny, nz = 3, 2, 1
+    N = nx * ny * nz
+    b1 = np.ones(N, np.int32)
+    z = np.zeros(N, np.int32)
+    sdf = np.full(N, -1.0, np.float32)
+    geo.sdf_near_bwalls(sdf, b1, z, z, nx, ny, nz, dr=3.0)
+    # from i=0: layers i < nx-1 -> 0, 1; from i=nx-1: cells nx-1-d > 0 -> 2 (d=0), 1 (d=1)
+    assert sdf.reshape(ny, nx)[0].tolist() == [0.5, 1.5, 0.5]
