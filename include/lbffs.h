@@ -141,6 +141,13 @@ int lbffs_download(lbffs_ctx *ctx, int field, void *host, size_t bytes);
 #define LBFFS_REDUCE_N 8
 int lbffs_reduce(lbffs_ctx *ctx, double out[LBFFS_REDUCE_N]);
 
+/* Density and velocity of every cell from the current populations, as the host computes
+ * them for an output step (cal_rhoUVW, D3Q19_opencl/D3Q19.hpp:647-666 and
+ * D3Q27_opencl/D3Q27.hpp:28-47, called at write.hpp:27): sums in direction order,
+ * u = sum(f cx)/rho + 0.5 dpdx/rho (lattice units).  Four arrays of nx*ny*nz host reals,
+ * `bytes` each; bit-identical to the reference host for FP32 storage. */
+int lbffs_output_macros(lbffs_ctx *ctx, float dpdx, void *rho, void *u, void *v, void *w, size_t bytes);
+
 /* Launch options of the step (no counterpart in the reference, whose work-group size
  * is left to the OpenCL runtime, ns.cpp:516). */
 enum {

@@ -1,7 +1,8 @@
 """lbffs_amd -- MI355X lattice-Boltzmann time-step engine for the LBFFS solver.
 
 Python view of the C ABI in ``include/lbffs.h`` (device step), ``include/lbffs_geom.h``
-(geometry preprocessing on the device) and ``include/lbffs_case.h`` (host case setup).  The library ``lbffs_amd/liblbffs.so`` is
+(geometry preprocessing on the device), ``include/lbffs_case.h`` (host case setup) and
+``include/lbffs_output.h`` (output step: VTK and restart files, diagnostics).  The library ``lbffs_amd/liblbffs.so`` is
 built by ``python -m lbffs_amd.build``; importing this package never builds it and
 never falls back to another implementation: without the library every call raises.
 """
@@ -117,13 +118,22 @@ def lib():
         L.lbffs_geom_solid.argtypes = [ip, vp, vp, vp, ip, ip, ip, ip, vp]
         L.lbffs_geom_sdf.argtypes = [ip, vp, vp, fl, fl, fl, ip, ip, ip, vp, vp]
         L.lbffs_case_sdf_near_bwalls.argtypes = [fl, vp, vp, vp, ip, ip, ip, vp]
+        L.lbffs_output_macros.argtypes = [vp, fl, vp, vp, vp, vp, sz]
+        L.lbffs_out_write_vtk.argtypes = [C.c_char_p, vp, ip]
+        L.lbffs_out_write_restart.argtypes = [C.c_char_p, ip, ip, ip, ip, ip, vp]
+        L.lbffs_out_read_restart.argtypes = [C.c_char_p, ip] + [C.POINTER(ip)] * 4 + [vp, sz]
+        L.lbffs_out_stats_compute.argtypes = [vp, vp]
+        L.lbffs_out_coeffs_fp64.argtypes = [vp, fl, fl, fl, fl, vp]
+        L.lbffs_out_accumulate_mean.argtypes = [C.c_long, ip, vp, vp, vp, vp, vp, vp, vp]
         for n in ("lbffs_create", "lbffs_upload", "lbffs_download", "lbffs_step", "lbffs_reduce",
                   "lbffs_synchronize", "lbffs_last_step_ms", "lbffs_destroy", "lbffs_case_units",
                   "lbffs_case_faces", "lbffs_case_init_feq", "lbffs_case_neisolid", "lbffs_case_outlet_sponge",
                   "lbffs_probe_copy", "lbffs_probe_bandwidth", "lbffs_autotune", "lbffs_set_launch",
                   "lbffs_ibm_set_markers", "lbffs_ibm_set_motion", "lbffs_ibm_download", "lbffs_batch_create",
                   "lbffs_batch_context", "lbffs_batch_set_launch", "lbffs_batch_step", "lbffs_batch_last_step_ms",
-                  "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf", "lbffs_case_sdf_near_bwalls"):
+                  "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf", "lbffs_case_sdf_near_bwalls",
+                  "lbffs_output_macros", "lbffs_out_write_vtk", "lbffs_out_write_restart", "lbffs_out_read_restart",
+                  "lbffs_out_stats_compute", "lbffs_out_coeffs_fp64", "lbffs_out_accumulate_mean"):
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib
@@ -240,6 +250,15 @@ class Lattice:
         out = (C.c_double * REDUCE_N)()
         _check(lib().lbffs_reduce(self._h, out), "lbffs_reduce")
         return np.array(out[:], dtype=np.float64)
+
+    def output_macros(self, dpdx=0.0):
+        """rho, u, v, w from the current populations, as the host computes them at an
+        output step (cal_rhoUVW, D3Q19_opencl/D3Q19.hpp:647-666)."""
+        dt = self._dtype("rho")
+        arrs = [np.empty(self.N, dt) for _ in range(4)]
+        _check(lib().lbffs_output_macros(self._h, float(dpdx), *[_ptr(a) for a in arrs], arrs[0].nbytes),
+               "lbffs_output_macros")
+        return dict(zip(("rho", "u", "v", "w"), arrs))
 
     def synchronize(self):
         _check(lib().lbffs_synchronize(self._h), "lbffs_synchronize")

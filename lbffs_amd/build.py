@@ -3,8 +3,9 @@
     python -m lbffs_amd.build          # product library lbffs_amd/liblbffs.so
 
 The product is one shared library, ``lbffs_amd/liblbffs.so``, holding the C ABI of
-``include/lbffs.h`` (device step, HIP kernels) and ``include/lbffs_case.h`` (host case
-setup).  Host case code is compiled by g++ with FMA contraction off so that its float
+``include/lbffs.h`` (device step, HIP kernels), ``include/lbffs_geom.h`` (geometry
+kernels), ``include/lbffs_case.h`` (host case setup) and ``include/lbffs_output.h``
+(output and restart files).  Host case code is compiled by g++ with FMA contraction off so that its float
 results match the reference host; device code is compiled by hipcc for gfx950 only.
 """
 from __future__ import annotations
@@ -43,16 +44,19 @@ def _run(cmd, verbose):
 
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    case_o = os.path.join(BUILD, "lbffs_case.o")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
-          "-c", os.path.join(CSRC, "lbffs_case.cpp"), "-o", case_o], verbose)
+    host_objs = []
+    for src in ("lbffs_case.cpp", "lbffs_output.cpp"):
+        o = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
+              "-c", os.path.join(CSRC, src), "-o", o], verbose)
+        host_objs.append(o)
     tmp = LIB + ".tmp"
     objs = []
     for src in ("lbffs_api.hip", "lbffs_probe.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o], verbose)
         objs.append(o)
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, case_o, "-o", tmp], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, *host_objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

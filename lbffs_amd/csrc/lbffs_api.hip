@@ -176,6 +176,39 @@ __global__ void decode_kernel(uint32_t N, size_t fs, const S *src, Real *dst)
         dst[(size_t)q * N + ic] = Codec<S, Real, Q>::load(src + (size_t)q * fs + ic, q, wq<Real, Q>(q));
 }
 
+// density and velocity from the current populations, as the host computes them at an
+// output step (cal_rhoUVW of D3Q19_opencl/D3Q19.hpp:647-666 and D3Q27_opencl/D3Q27.hpp:28-47):
+// FP32 sums in direction order of f c (the products by 0 kept: they decide the sign of a
+// zero), u /= rho, then u += (0.5 dpdx) / rho
+template <typename S, typename Real, int Q>
+__global__ void output_macros_kernel(uint32_t N, size_t fs, const S *f, Real dpdx, Real *rho, Real *u, Real *v,
+                                     Real *w)
+{
+#pragma clang fp contract(off)
+    const uint32_t ic = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic >= N) return;
+    Real r = 0, mx = 0, my = 0, mz = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const Real fq = Codec<S, Real, Q>::load(f + (size_t)q * fs + ic, q, wq<Real, Q>(q));
+        const Real cx = Real(Q == 27 ? L27::cx[q] : L19::cx[q]);
+        const Real cy = Real(Q == 27 ? L27::cy[q] : L19::cy[q]);
+        const Real cz = Real(Q == 27 ? L27::cz[q] : L19::cz[q]);
+        r += fq;
+        mx += fq * cx;
+        my += fq * cy;
+        mz += fq * cz;
+    }
+    mx /= r;
+    my /= r;
+    mz /= r;
+    mx += Real(0.5) * dpdx / r;
+    rho[ic] = r;
+    u[ic] = mx;
+    v[ic] = my;
+    w[ic] = mz;
+}
+
 __device__ inline double wave_sum(double x)
 {
 #pragma unroll
@@ -640,6 +673,33 @@ struct ReduceOp {
     static int run(lbffs_ctx *c, double *o) { return reduce_impl<S, Real>(c, o); }
 };
 
+template <typename S, typename Real>
+int output_macros_impl(lbffs_ctx *c, double dpdx, void *const *host)
+{
+    const size_t n = (size_t)c->N;
+    void *tmp = nullptr;
+    if (int e = dalloc(&tmp, 4 * n * sizeof(Real))) return e;
+    Real *t = (Real *)tmp;
+    if (c->d.q == 27)
+        hipLaunchKernelGGL((output_macros_kernel<S, Real, 27>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
+                           (uint32_t)c->N, c->fstride, (const S *)c->f[c->cur], (Real)dpdx, t, t + n, t + 2 * n,
+                           t + 3 * n);
+    else
+        hipLaunchKernelGGL((output_macros_kernel<S, Real, 19>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
+                           (uint32_t)c->N, c->fstride, (const S *)c->f[c->cur], (Real)dpdx, t, t + n, t + 2 * n,
+                           t + 3 * n);
+    hipError_t e1 = hipGetLastError();
+    if (e1 == hipSuccess) e1 = hipStreamSynchronize(c->stream);
+    for (int a = 0; a < 4 && e1 == hipSuccess; a++) e1 = hipMemcpy(host[a], t + a * n, n * sizeof(Real), hipMemcpyDeviceToHost);
+    hipFree(tmp);
+    if (e1 != hipSuccess) return fail(LBFFS_E_DEVICE, std::string("output macros: ") + hipGetErrorString(e1));
+    return 0;
+}
+template <typename S, typename Real>
+struct OutputMacrosOp {
+    static int run(lbffs_ctx *c, double dpdx, void *const *h) { return output_macros_impl<S, Real>(c, dpdx, h); }
+};
+
 void free_all(lbffs_ctx *c)
 {
     void *ptrs[] = {c->f[0], c->f[1], c->boundary, c->b1, c->b2, c->b3, c->solid, c->neiSolid, c->sdf, c->u0, c->v0,
@@ -1088,6 +1148,18 @@ int lbffs_reduce(lbffs_ctx *c, double out[LBFFS_REDUCE_N])
         if (e) return e;
     }
     return dispatch<ReduceOp>(c, out);
+}
+
+int lbffs_output_macros(lbffs_ctx *c, float dpdx, void *rho, void *u, void *v, void *w, size_t bytes)
+{
+    if (!c || !rho || !u || !v || !w) return fail(LBFFS_E_INVALID, "null argument");
+    if (!c->f_loaded) return fail(LBFFS_E_STATE, "populations not uploaded");
+    if (bytes != (size_t)c->N * (size_t)c->real_size)
+        return fail(LBFFS_E_INVALID, "output_macros size " + std::to_string(bytes) + " != " +
+                                         std::to_string((size_t)c->N * (size_t)c->real_size));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    void *const h[4] = {rho, u, v, w};
+    return dispatch<OutputMacrosOp>(c, (double)dpdx, h);
 }
 
 int lbffs_synchronize(lbffs_ctx *c)

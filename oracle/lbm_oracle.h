@@ -101,6 +101,9 @@ int orc_geom_solid(int n, const float *v0, const float *v1, const float *v2, int
 int orc_geom_sdf(int n, const float *c, const float *nrm, float sdfIni, float dr, float p, int nx, int ny, int nz,
                  const unsigned char *neiSolid, float *sdf);
 
+/* output step (output_oracle.c): cal_rhoUVW, D3Q19_opencl/D3Q19.hpp:647-666 */
+void orc_cal_rhouvw(int Q, long N, const float *f, float dpdx, float *rho, float *u, float *v, float *w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,7 +83,17 @@ def _bind(L):
     L.orc_geom_is_cross.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int]
     L.orc_geom_solid.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
     L.orc_geom_sdf.argtypes = [C.c_int, vp, vp, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, vp, vp]
+    L.orc_cal_rhouvw.argtypes = [C.c_int, C.c_long, vp, C.c_float, vp, vp, vp, vp]
     return L
+
+
+def cal_rhouvw(q, f, dpdx=0.0):
+    """Restated host cal_rhoUVW (D3Q19.hpp:647-666): dict of rho, u, v, w (float32)."""
+    f = np.ascontiguousarray(f, np.float32)
+    N = f.size // q
+    out = {k: np.empty(N, np.float32) for k in ("rho", "u", "v", "w")}
+    orc().orc_cal_rhouvw(q, N, _p(f), dpdx, *[_p(out[k]) for k in ("rho", "u", "v", "w")])
+    return out
 
 
 def geom_solid(v0, v1, v2, nx, ny, nz, invert=False, solid=None):
