@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the fused D3Q19 time step on one MI355X, in the reference's metric.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload backstep|cylinder|cavity]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload backstep|cylinder|cavity|box27|ensemble]
 
 Default workload: backward-facing step, D3Q19, Smagorinsky LES (Cs=0.1, wall damping),
 1536 x 192 x 128 lattice (37.7M cells), FixedVelocity inlet with seeded 5 % Gaussian
@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="backstep", choices=["backstep", "cylinder", "cavity", "ensemble"])
+    ap.add_argument("--workload", default="backstep", choices=["backstep", "cylinder", "cavity", "box27", "ensemble"])
     ap.add_argument("--domains", type=int, default=512, help="ensemble: number of lattices")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
@@ -52,6 +52,9 @@ def build_case(name):
         return cases.backward_step()
     if name == "cylinder":
         return cases.cylinder_channel()
+    if name == "box27":
+        # D3Q27 cumulant step on a periodic 250^3 box (15.6M cells), no sub-grid model
+        return cases.periodic_box(250, 250, 250, tau=0.8, sgs=lb.SGS_NONE, q=27)
     return cases.cavity(64)
 
 
@@ -61,7 +64,9 @@ def log(*a):
 
 def run_form(case, storage, warmup, steps, tune=True):
     """Warm up, then time exactly `steps` steps bracketed by device synchronisation."""
-    lat = lb.Lattice(case.nx, case.ny, case.nz, storage=storage, sgs=case.sgs, omega0=case.omega0)
+    coll = lb.COLL_CUMULANT if case.q == 27 else lb.COLL_RR
+    lat = lb.Lattice(case.nx, case.ny, case.nz, q=case.q, storage=storage, sgs=case.sgs, collision=coll,
+                     omega0=case.omega0)
     case.upload_to(lat)
     lat.tuned = lat.autotune(10, **case.params) if tune else None
     if warmup:
@@ -203,7 +208,7 @@ def main():
     for name in run:
         log(f"[bench] {name}: warmup {a.warmup}, timed {a.steps}")
         lat, wall, dev_ms = run_form(case, forms[name], a.warmup, a.steps)
-        bytes_per_cell = 2 * 19 * (4 if name == "f32" else 2)
+        bytes_per_cell = 2 * case.q * (4 if name == "f32" else 2)
         launch_ms = dev_ms / a.steps
         res[name] = {"wall_s": wall, "dev_ms": dev_ms, "launch_ms": launch_ms, "launch": lat.tuned,
                      "mlups": a.steps * N / wall / 1e6, "bytes_per_cell": bytes_per_cell,
@@ -248,15 +253,17 @@ def main():
         "data": "synthetic",
         "config": {"workload": {"backstep": "backward-facing step D3Q19 Smagorinsky LES Re_H=5500",
                                 "cylinder": "channel past cylinder D3Q19 Re=100",
-                                "cavity": "lid-driven cavity D3Q19 Re=100"}[a.workload],
+                                "cavity": "lid-driven cavity D3Q19 Re=100",
+                                "box27": "periodic box D3Q27 cumulant, no SGS"}[a.workload],
                    "lattice": [case.nx, case.ny, case.nz], "cells": N, "fluid_cells": fluid,
-                   "collision": "recursive-regularized",
+                   "collision": "cumulant" if case.q == 27 else "recursive-regularized",
                    "sgs": {0: "WALE", 1: "Smagorinsky", 2: "none"}[case.sgs],
                    "storage": main_form, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_Bps"] / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(r["achieved_Bps"] / HBM_PEAK, 4),
                      "traffic": traffic,
-                     "bytes_per_cell_update": r["bytes_per_cell"], "kernel": "step19_kernel",
+                     "bytes_per_cell_update": r["bytes_per_cell"],
+                     "kernel": "step27_kernel" if case.q == 27 else "step19_kernel",
                      "launch_ms": round(r["launch_ms"], 4), "probes_GBps": probes},
         "forms": {k: {"mlups": round(v["mlups"], 1), "launch_ms": round(v["launch_ms"], 4),
                       "hbm_frac": round(v["achieved_Bps"] / HBM_PEAK, 4),
