@@ -151,7 +151,7 @@ int lbffs_output_macros(lbffs_ctx *ctx, float dpdx, void *rho, void *u, void *v,
 /* Launch options of the step (no counterpart in the reference, whose work-group size
  * is left to the OpenCL runtime, ns.cpp:516). */
 enum {
-    LBFFS_TUNE_NT_STORE = 1, /* write new populations with non-temporal stores */
+    LBFFS_TUNE_NT_STORE = 1, /* accepted, no effect: new populations are always stored non-temporally */
     LBFFS_TUNE_NO_XCD = 2    /* keep the hardware work-group order across the 8 XCDs */
 };
 

@@ -68,7 +68,7 @@ using Step19Args = StepArgs<Real, 19>;
 
 // launch options (uniform per launch)
 enum : int {
-    TUNE_NT_STORE = 1,  // write the new populations with non-temporal stores
+    TUNE_NT_STORE = 1,  // accepted, no effect: the populations are always stored non-temporally
     TUNE_NO_XCD = 2,    // keep the hardware work-group order (no XCD slab remap)
 };
 

@@ -874,6 +874,13 @@ int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out)
         return fail(LBFFS_E_UNSUPPORTED, "D3Q27 runs the cumulant collision (K27:2824)");
     if (desc->block != 0 && (desc->block % 64 != 0 || desc->block > 256 || desc->block < 64))
         return fail(LBFFS_E_INVALID, "block must be 64, 128, 192 or 256");
+    // the step kernels address a population array with 32-bit byte offsets from its base
+    {
+        const long elem = (desc->storage == LBFFS_STORE_F64) ? 8 : (desc->storage == LBFFS_STORE_F32 ? 4 : 2);
+        if (N * elem > 0xFFFFFFFFL)
+            return fail(LBFFS_E_INVALID, "one population array must stay below 4 GiB (" + std::to_string(N) +
+                                             " cells of " + std::to_string(elem) + " bytes)");
+    }
 
     lbffs_ctx *c = new lbffs_ctx();
     c->d = *desc;
@@ -1011,11 +1018,11 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
         he = hipMemcpyAsync((char *)saved_r + k * n * rs, keep[k], n * rs, hipMemcpyDeviceToDevice, c->stream);
     if (saved_m && he == hipSuccess) he = hipMemcpyAsync(saved_m, c->markers, mbytes, hipMemcpyDeviceToDevice, c->stream);
     const int blocks[3] = {64, 128, 256};
-    const int tunes[4] = {0, TUNE_NT_STORE, TUNE_NO_XCD, TUNE_NT_STORE | TUNE_NO_XCD};
+    const int tunes[2] = {0, TUNE_NO_XCD};
     float bms = 1e30f;
     int bb = block0, bt = tune0;
     for (int bi = 0; bi < 3 && he == hipSuccess && rc == 0; bi++) {
-        for (int ti = 0; ti < 4 && he == hipSuccess && rc == 0; ti++) {
+        for (int ti = 0; ti < 2 && he == hipSuccess && rc == 0; ti++) {
             c->block = blocks[bi];
             c->tune = tunes[ti];
             rc = dispatch<StepOp>(c, 1, p);  // warm the shape
