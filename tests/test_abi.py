@@ -60,6 +60,10 @@ def test_strerror_and_invalid_arguments():
     assert L.lbffs_batch_step(None, 1, None) == -1
     assert L.lbffs_batch_destroy(None) == 0
     assert L.lbffs_upload(None, 0, None, 0) == -1
+    # a population array addressed with 32-bit byte offsets: 1100^3 FP32 cells are refused
+    d = lb.Desc(abi_version=lb.ABI_VERSION, q=19, nx=1100, ny=1100, nz=1100, storage=lb.STORE_F32)
+    assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"4 GiB" in L.lbffs_last_error()
     # geometry: argument checks come before any device call
     assert L.lbffs_geom_solid(1, None, None, None, 4, 4, 4, 0, None) == -1
     assert L.lbffs_geom_sdf(1, None, None, 1.0, 1.0, 7.0, 4, 4, 4, None, None) == -1
