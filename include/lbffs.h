@@ -85,7 +85,12 @@ enum {
     LBFFS_FIELD_GX = 20,    /* N host reals: IBM forcing density */
     LBFFS_FIELD_GY = 21,
     LBFFS_FIELD_GZ = 22,
-    LBFFS_FIELD_COUNT = 23
+    LBFFS_FIELD_QLINK = 23, /* Q*N host reals, q-major: q-distance of each solid link (fraction
+                               of the link from the fluid cell to the wall), read by the
+                               interpolated bounce-back when lbffs_step_params.ibb = 1 (the
+                               layout of the reference's unused qf array, ns.cpp:215-217);
+                               device memory is allocated on the first upload */
+    LBFFS_FIELD_COUNT = 24
 };
 
 /* Fixed per-run description of a lattice.  Replaces the program build and buffer
@@ -115,7 +120,11 @@ typedef struct {
                         (reference semantics throughout); 0: only on the last step of
                         the call and for the cells later steps read */
     float clim;      /* limiter of the D3Q27 cumulant collision (input.hpp default 0.01) */
-    int reserved[7];
+    int ibb;         /* 0: the reference's solid links (D3Q19 half-way bounce-back, D3Q27
+                        interpolated with q from the signed distances); 1: Filippova-Hanel /
+                        Mei-Luo-Shyy interpolated bounce-back (the formula of K27:1846-1888) on
+                        both lattices with q from LBFFS_FIELD_QLINK */
+    int reserved[6];
 } lbffs_step_params;
 
 typedef struct lbffs_ctx lbffs_ctx;

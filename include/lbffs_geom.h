@@ -41,6 +41,18 @@ int lbffs_geom_solid(int n, const float *v0, const float *v1, const float *v2, i
 int lbffs_geom_sdf(int n, const float *centroid, const float *normal, float sdf_ini, float dr, float p, int nx, int ny,
                    int nz, const uint8_t *neiSolid, float *sdf);
 
+/* q-distances of the solid links for the interpolated bounce-back (LBFFS_FIELD_QLINK of
+ * lbffs.h), from the surface in FP64.  No counterpart runs in the reference, whose setQf
+ * (walls.hpp:553-584) derives q from the signed distances and is never uploaded.  For every
+ * cell with neiSolid == 1 and solid == 0 and every direction c_d whose upwind cell x - c_d
+ * (periodic wrap, as setNeiSolid) is solid, q is the smallest t in [0, 1] at which the
+ * segment x - t c_d meets one of the n triangles (Moller-Trumbore in FP64, vertices as
+ * given: no periodic images); qlink[d*N + ic] = (float)q.  Links that meet no triangle keep
+ * their value and are counted in *misses (may be NULL).  qlink: q*nx*ny*nz floats, in and
+ * out, q-major. */
+int lbffs_geom_qlinks(int q, int n, const float *v0, const float *v1, const float *v2, int nx, int ny, int nz,
+                      const uint8_t *solid, const uint8_t *neiSolid, float *qlink, int *misses);
+
 #ifdef __cplusplus
 }
 #endif

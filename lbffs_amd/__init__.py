@@ -29,7 +29,7 @@ BC = {"Cyclic": 0, "BounceBack": 1, "Outlet": 3, "Symmetry": 4, "FixedVelocity":
 
 FIELDS = ["f", "boundary", "boundary1", "boundary2", "boundary3", "sdf", "solid", "neiSolid",
           "u0", "v0", "w0", "Fwx", "Fwy", "Fwz", "tauSGS", "rho", "u", "v", "w", "omega",
-          "GxIBM", "GyIBM", "GzIBM"]
+          "GxIBM", "GyIBM", "GzIBM", "qlink"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 INT_FIELDS = {"boundary", "boundary1", "boundary2", "boundary3"}
 U8_FIELDS = {"solid", "neiSolid"}
@@ -49,7 +49,7 @@ class Desc(C.Structure):
 class StepParams(C.Structure):
     _fields_ = [("dpdx", C.c_float), ("LES", C.c_float), ("omegaB", C.c_float), ("spzWidth", C.c_float),
                 ("Cs", C.c_float), ("damping", C.c_int), ("store_every", C.c_int), ("clim", C.c_float),
-                ("reserved", C.c_int * 7)]
+                ("ibb", C.c_int), ("reserved", C.c_int * 6)]
 
 
 class IbmMotion(C.Structure):
@@ -117,6 +117,7 @@ def lib():
         fl = C.c_float
         L.lbffs_geom_solid.argtypes = [ip, vp, vp, vp, ip, ip, ip, ip, vp]
         L.lbffs_geom_sdf.argtypes = [ip, vp, vp, fl, fl, fl, ip, ip, ip, vp, vp]
+        L.lbffs_geom_qlinks.argtypes = [ip, ip, vp, vp, vp, ip, ip, ip, vp, vp, vp, C.POINTER(ip)]
         L.lbffs_case_sdf_near_bwalls.argtypes = [fl, vp, vp, vp, ip, ip, ip, vp]
         L.lbffs_output_macros.argtypes = [vp, fl, vp, vp, vp, vp, sz]
         L.lbffs_out_write_vtk.argtypes = [C.c_char_p, vp, ip]
@@ -131,7 +132,7 @@ def lib():
                   "lbffs_probe_copy", "lbffs_probe_bandwidth", "lbffs_autotune", "lbffs_set_launch",
                   "lbffs_ibm_set_markers", "lbffs_ibm_set_motion", "lbffs_ibm_download", "lbffs_batch_create",
                   "lbffs_batch_context", "lbffs_batch_set_launch", "lbffs_batch_step", "lbffs_batch_last_step_ms",
-                  "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf", "lbffs_case_sdf_near_bwalls",
+                  "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf", "lbffs_geom_qlinks", "lbffs_case_sdf_near_bwalls",
                   "lbffs_output_macros", "lbffs_out_write_vtk", "lbffs_out_write_restart", "lbffs_out_read_restart",
                   "lbffs_out_stats_compute", "lbffs_out_coeffs_fp64", "lbffs_out_accumulate_mean"):
             getattr(L, n).restype = C.c_int
@@ -149,9 +150,9 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def _params(dpdx=0.0, LES=0.0, omegaB=0.1, spzWidth=0.1, Cs=0.1, damping=1, store_every=0, clim=0.01):
+def _params(dpdx=0.0, LES=0.0, omegaB=0.1, spzWidth=0.1, Cs=0.1, damping=1, store_every=0, clim=0.01, ibb=0):
     return StepParams(dpdx=dpdx, LES=LES, omegaB=omegaB, spzWidth=spzWidth, Cs=Cs, damping=damping,
-                      store_every=store_every, clim=clim)
+                      store_every=store_every, clim=clim, ibb=int(ibb))
 
 
 def probe_copy(nbytes, reps=20):
@@ -207,7 +208,7 @@ class Lattice:
         _check(lib().lbffs_upload(self._h, FIELD_ID[name], _ptr(a), a.nbytes), f"upload {name}")
 
     def download(self, name):
-        n = self.q * self.N if name == "f" else self.N
+        n = self.q * self.N if name in ("f", "qlink") else self.N
         a = np.empty(n, dtype=self._dtype(name))
         _check(lib().lbffs_download(self._h, FIELD_ID[name], _ptr(a), a.nbytes), f"download {name}")
         return a

@@ -145,12 +145,53 @@ def channel(nx=8, ny=20, nz=1, tau=0.8, force=1e-5, sgs=SGS_NONE, q=19):
     return make_case("channel", nx, ny, nz, 1.0, 1.0, U0, nu, codes, dpdx=dpdx, sgs=sgs, q=q)
 
 
+def cylinder_qlinks(q, nx, ny, nz, solid, cx, cy, R):
+    """q-distances of the links that leave a fluid cell towards a solid cell of the cylinder
+    (axis along z through (cx, cy), radius R), in FP64: the link from the fluid node x to
+    x - c_q meets the circle at x - t c_q with t the smaller root of |p - t d|^2 = R^2
+    (p = x - centre, d = c_q in the x-y plane); q = t.  Q*N float32, q-major, 0.5 where no
+    solid link (the reference's qfIni, ns.cpp:215)."""
+    from . import BC  # noqa: F401  (keeps the import list of this module in one place)
+    cxs, cys = (np.array(v) for v in _dirs(q)[:2])
+    N = nx * ny * nz
+    out = np.full((q, N), 0.5, np.float64)
+    sol = np.asarray(solid, np.uint8).reshape(nz, ny, nx).astype(bool)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    px, py = i - float(cx), j - float(cy)
+    fluid = ~sol
+    for d in range(1, q):
+        dx, dy = float(cxs[d]), float(cys[d])
+        if dx == 0.0 and dy == 0.0:
+            continue
+        # the upwind cell x - c_q (periodic wrap as setNeiSolid) must be solid
+        up = np.roll(np.roll(sol, int(cys[d]), axis=1), int(cxs[d]), axis=2)
+        m = fluid & up
+        a = dx * dx + dy * dy
+        b = -(px * dx + py * dy)
+        cc = px * px + py * py - R * R
+        disc = np.maximum(b * b - a * cc, 0.0)
+        t = (-b - np.sqrt(disc)) / a
+        out[d][m.reshape(-1)] = np.clip(t, 0.0, 1.0)[m]
+    return out.astype(np.float32).reshape(-1)
+
+
+def _dirs(q):
+    c19 = ([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0],
+           [0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1],
+           [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1])
+    if q == 19:
+        return c19
+    return (c19[0] + [1, -1, -1, 1, 1, -1, 1, -1], c19[1] + [1, -1, 1, -1, -1, 1, 1, -1],
+            c19[2] + [1, -1, 1, -1, 1, -1, -1, 1])
+
+
 def cylinder_channel(nx=400, ny=160, nz=16, D=40, cx=100, cy=80, u_in=0.05, noise=1e-3, Re=100.0,
-                     spz_cells=20, sgs=SGS_WALE, seed=SEED, q=19, sdf_from_geometry=False):
+                     spz_cells=20, sgs=SGS_WALE, seed=SEED, q=19, sdf_from_geometry=False, ibb=False):
     """Channel flow past a circular cylinder: FixedVelocity inlet (u_in plus seeded uniform
     noise per inlet cell), Outlet (Geier convective) at nxMax with a kernel sponge of
     spz_cells, bounce-back walls in y, periodic z; cylinder cells (i-cx)^2+(j-cy)^2 < (D/2)^2
-    are solid."""
+    are solid.  ibb=True adds analytic q-distances (FP64) and selects the interpolated
+    bounce-back on the cylinder."""
     U0 = u_in
     nu_lat = u_in * D / Re
     nu = nu_lat * (1.0 / (nx - 1)) * (1.0 / U0)
@@ -174,6 +215,10 @@ def cylinder_channel(nx=400, ny=160, nz=16, D=40, cx=100, cy=80, u_in=0.05, nois
     u0 = c.arrays["u0"]
     u0[inlet] = (u0[inlet].astype(np.float64)
                  + rng.uniform(-noise, noise, size=int(inlet.sum()))).astype(np.float32)
+    if ibb:
+        # Filippova-Hanel interpolated bounce-back with analytic q-distances
+        c.arrays["qlink"] = cylinder_qlinks(q, nx, ny, nz, solid, cx, cy, D / 2.0)
+        c.params["ibb"] = 1
     c.meta.update(D=D, cx=cx, cy=cy, u_in=u_in)
     return c
 

@@ -55,6 +55,8 @@ struct StepArgs {
     Real *tau, *Fwx, *Fwy, *Fwz, *Gx, *Gy, *Gz;
     Real dpdx, LES, omegaB, spz, Cs, omega0;
     Real clim;  // cumulant limiter (D3Q27)
+    const Real *qlink;  // q-distances of the solid links (Q*N, q-major), read when ibb
+    int ibb;            // interpolated bounce-back with qlink instead of the reference's walls
     int damping;
     int mac_all;      // write rho/u/v/w for every fluid cell
     int aux_all;      // write tauSGS / Fw / G for every fluid cell

@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "d3q27_step.hpp"
+
 namespace lbffs {
 
 // one triangle as the crossing test reads it (vertices, then the y/z extent of its box)
@@ -148,6 +150,81 @@ __global__ void __launch_bounds__(256)
         }
     }
     sdf[ic] = sumD != 0.f ? s / sumD : sdf_ini;
+}
+
+// ---- q-distances of the solid links from the surface, in FP64 ----
+//
+// For a fluid cell x with neiSolid == 1 and a direction c_d whose upwind cell x - c_d is
+// solid, q is the smallest t in [0, 1] at which the segment x - t c_d meets a triangle
+// (Moller-Trumbore in FP64, contraction off: the result is a function of the triangle and
+// the link alone, so the order in which candidates are visited does not matter).  The
+// triangles come binned on the host into cubic buckets of kQBucket cells: a triangle is
+// listed in every bucket its box, grown by one cell, overlaps, so every triangle that can
+// meet a link of x is listed in the bucket of x.
+
+constexpr int kQBucket = 4;
+
+struct GeomTri64 {
+    double v0[3], e1[3], e2[3];
+};
+
+// smallest t in [0, 1] where x - t c meets the triangle, or 2 (no hit)
+__device__ inline double geom_link_hit(const GeomTri64 &T, const double *x, const double *c)
+{
+#pragma clang fp contract(off)
+    const double dir[3] = {-c[0], -c[1], -c[2]};
+    const double p[3] = {dir[1] * T.e2[2] - dir[2] * T.e2[1], dir[2] * T.e2[0] - dir[0] * T.e2[2],
+                         dir[0] * T.e2[1] - dir[1] * T.e2[0]};
+    const double det = T.e1[0] * p[0] + T.e1[1] * p[1] + T.e1[2] * p[2];
+    if (fabs(det) < 1e-14) return 2.0;  // link parallel to the triangle's plane
+    const double inv = 1.0 / det;
+    const double sv[3] = {x[0] - T.v0[0], x[1] - T.v0[1], x[2] - T.v0[2]};
+    const double u = (sv[0] * p[0] + sv[1] * p[1] + sv[2] * p[2]) * inv;
+    if (u < -1e-12 || u > 1.0 + 1e-12) return 2.0;
+    const double qv[3] = {sv[1] * T.e1[2] - sv[2] * T.e1[1], sv[2] * T.e1[0] - sv[0] * T.e1[2],
+                          sv[0] * T.e1[1] - sv[1] * T.e1[0]};
+    const double v = (dir[0] * qv[0] + dir[1] * qv[1] + dir[2] * qv[2]) * inv;
+    if (v < -1e-12 || u + v > 1.0 + 1e-12) return 2.0;
+    const double t = (T.e2[0] * qv[0] + T.e2[1] * qv[1] + T.e2[2] * qv[2]) * inv;
+    return (t >= 0.0 && t <= 1.0) ? t : 2.0;
+}
+
+// one work-item per listed neiSolid cell; Q directions; q written for the solid links
+// found (no hit keeps the caller's value); misses counted
+template <int Q>
+__global__ void __launch_bounds__(256)
+    geom_qlinks_kernel(const uint32_t *__restrict__ cells, uint32_t ncells, const GeomTri64 *__restrict__ tris,
+                       const int *__restrict__ bstart, const int *__restrict__ btri, int bx, int by, int bz, int nx,
+                       int ny, int nz, const uint8_t *__restrict__ solid, float *qlink, uint32_t *misses)
+{
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ncells) return;
+    const uint32_t ic = cells[idx];
+    const int i = (int)(ic % (uint32_t)nx), j = (int)(ic / (uint32_t)nx % (uint32_t)ny);
+    const int k = (int)(ic / ((uint32_t)nx * (uint32_t)ny));
+    const double x[3] = {(double)i, (double)j, (double)k};
+    const int bi = i / kQBucket, bj = j / kQBucket, bk = k / kQBucket;
+    const int b = bi + bx * (bj + by * bk);
+    const int t0 = bstart[b], t1 = bstart[b + 1];
+    const size_t N = (size_t)nx * ny * nz;
+    for (int d = 1; d < Q; d++) {
+        const int cx = Q == 27 ? L27::cx[d] : L19::cx[d];
+        const int cy = Q == 27 ? L27::cy[d] : L19::cy[d];
+        const int cz = Q == 27 ? L27::cz[d] : L19::cz[d];
+        // upwind cell with periodic wrap, as setNeiSolid (walls.hpp:187-216)
+        int si = i - cx, sj = j - cy, sk = k - cz;
+        si = si < 0 ? nx - 1 : (si >= nx ? 0 : si);
+        sj = sj < 0 ? ny - 1 : (sj >= ny ? 0 : sj);
+        sk = sk < 0 ? nz - 1 : (sk >= nz ? 0 : sk);
+        if (solid[(size_t)si + (size_t)nx * ((size_t)sj + (size_t)ny * sk)] != 1) continue;
+        const double c[3] = {(double)cx, (double)cy, (double)cz};
+        double best = 2.0;
+        for (int t = t0; t < t1; t++) best = fmin(best, geom_link_hit(tris[btri[t]], x, c));
+        if (best <= 1.0)
+            qlink[(size_t)d * N + ic] = (float)best;
+        else
+            atomicAdd(misses, 1u);
+    }
 }
 
 }  // namespace lbffs

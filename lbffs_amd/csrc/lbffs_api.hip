@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -62,6 +63,7 @@ struct lbffs_ctx {
     uint8_t *solid = nullptr, *neiSolid = nullptr;
     void *sdf = nullptr, *u0 = nullptr, *v0 = nullptr, *w0 = nullptr, *omega = nullptr;
     void *Fwx = nullptr, *Fwy = nullptr, *Fwz = nullptr, *tau = nullptr;
+    void *qlink = nullptr;  // q-distances of the solid links (Q*N reals), allocated on upload
     void *Gx = nullptr, *Gy = nullptr, *Gz = nullptr;
     void *rho[2] = {nullptr, nullptr}, *u[2] = {nullptr, nullptr}, *v[2] = {nullptr, nullptr},
          *w[2] = {nullptr, nullptr};
@@ -137,16 +139,16 @@ __global__ void pack_flags_kernel(int Q, uint32_t N, int nx, int ny, int nz, Fas
         const bool on_wall = ((i == 0 || i == nx - 1) && is_wall_code(c1)) || ((j == 0 || j == ny - 1) && is_wall_code(c2)) ||
                              ((k == 0 || k == nz - 1) && is_wall_code(c3));
         if (!sol && bnd == 1 && neiSolid[ic] == 1 && on_wall) e |= PACK_NEISOLID_FACE;
-        // the interpolated bounce-back reads the previous-step moments of a NEISOLID cell
-        // and of its neighbours: keep them stored every step
-        for (int dz = -1; dz <= 1 && !keep; dz++)
-            for (int dy = -1; dy <= 1 && !keep; dy++)
-                for (int dx = -1; dx <= 1 && !keep; dx++) {
-                    const int ii = i + dx, jj = j + dy, kz = k + dz;
-                    if (ii < 0 || ii >= nx || jj < 0 || jj >= ny || kz < 0 || kz >= nz) continue;
-                    if (neiSolid[(long)ii + (long)nx * ((long)jj + (long)ny * kz)] == 1) keep = true;
-                }
     }
+    // the interpolated bounce-back (D3Q27 always, D3Q19 with ibb = 1) reads the
+    // previous-step moments of a NEISOLID cell and of its neighbours: keep them stored
+    for (int dz = -1; dz <= 1 && !keep; dz++)
+        for (int dy = -1; dy <= 1 && !keep; dy++)
+            for (int dx = -1; dx <= 1 && !keep; dx++) {
+                const int ii = i + dx, jj = j + dy, kz = k + dz;
+                if (ii < 0 || ii >= nx || jj < 0 || jj >= ny || kz < 0 || kz >= nz) continue;
+                if (neiSolid[(long)ii + (long)nx * ((long)jj + (long)ny * kz)] == 1) keep = true;
+            }
     if (keep) fl |= F_KEEPMAC;
     if (sdf[ic] != Real(10000)) fl |= F_HASSDF;
     if (omega[ic] != omega0) fl |= F_OMEGAVAR;
@@ -271,7 +273,8 @@ size_t field_host_bytes(const lbffs_ctx *c, int field)
 {
     const size_t N = (size_t)c->N;
     switch (field) {
-    case LBFFS_FIELD_F: return (size_t)c->d.q * N * c->real_size;
+    case LBFFS_FIELD_F:
+    case LBFFS_FIELD_QLINK: return (size_t)c->d.q * N * c->real_size;
     case LBFFS_FIELD_BOUNDARY:
     case LBFFS_FIELD_BOUNDARY1:
     case LBFFS_FIELD_BOUNDARY2:
@@ -303,6 +306,7 @@ void *field_ptr(lbffs_ctx *c, int field)
     case LBFFS_FIELD_U: return c->u[c->mcur];
     case LBFFS_FIELD_V: return c->v[c->mcur];
     case LBFFS_FIELD_W: return c->w[c->mcur];
+    case LBFFS_FIELD_QLINK: return c->qlink;
     case LBFFS_FIELD_OMEGA: return c->omega;
     case LBFFS_FIELD_GX: return c->Gx;
     case LBFFS_FIELD_GY: return c->Gy;
@@ -442,6 +446,8 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
     a.spz = (Real)p->spzWidth;
     a.Cs = (Real)p->Cs;
     a.clim = (Real)p->clim;
+    a.ibb = p->ibb ? 1 : 0;
+    a.qlink = (const Real *)c->qlink;
     a.omega0 = (Real)c->d.omega0;
     a.damping = p->damping;
     a.sponge_mask = c->sponge_mask;
@@ -705,7 +711,7 @@ void free_all(lbffs_ctx *c)
     void *ptrs[] = {c->f[0], c->f[1], c->boundary, c->b1, c->b2, c->b3, c->solid, c->neiSolid, c->sdf, c->u0, c->v0,
                     c->w0, c->omega, c->Fwx, c->Fwy, c->Fwz, c->tau, c->Gx, c->Gy, c->Gz, c->rho[0], c->rho[1],
                     c->u[0], c->u[1], c->v[0], c->v[1], c->w[0], c->w[1], c->flag, c->d_err, c->d_part, c->d_red,
-                    c->markers, c->Gm, c->mark, c->list, c->count};
+                    c->markers, c->Gm, c->mark, c->list, c->count, c->qlink};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -956,6 +962,9 @@ int lbffs_upload(lbffs_ctx *c, int field, const void *host, size_t bytes)
         if (!e) c->f_loaded = true;
         return e;
     }
+    if (field == LBFFS_FIELD_QLINK && !c->qlink) {
+        if (int e = dalloc(&c->qlink, bytes)) return e;
+    }
     HIP_TRY(hipMemcpy(field_ptr(c, field), host, bytes, hipMemcpyHostToDevice));
     if (field_affects_flags(field)) c->flags_dirty = true;
     if (field == LBFFS_FIELD_GX || field == LBFFS_FIELD_GY || field == LBFFS_FIELD_GZ) c->g_dirty = true;
@@ -967,6 +976,7 @@ int lbffs_step(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
     if (!c || !p) return fail(LBFFS_E_INVALID, "null argument");
     if (nsteps < 0) return fail(LBFFS_E_INVALID, "nsteps < 0");
     if (!c->f_loaded) return fail(LBFFS_E_STATE, "populations were never uploaded");
+    if (p->ibb && !c->qlink) return fail(LBFFS_E_STATE, "ibb = 1 needs the q-distances (LBFFS_FIELD_QLINK)");
     if (c->flags_dirty) {
         int e = is_f64(c) ? pack_flags<double>(c) : pack_flags<float>(c);
         if (e) return e;
@@ -986,6 +996,7 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
     if (!c || !p) return fail(LBFFS_E_INVALID, "null argument");
     if (trial_steps < 1) return fail(LBFFS_E_INVALID, "trial_steps < 1");
     if (!c->f_loaded) return fail(LBFFS_E_STATE, "populations were never uploaded");
+    if (p->ibb && !c->qlink) return fail(LBFFS_E_STATE, "ibb = 1 needs the q-distances (LBFFS_FIELD_QLINK)");
     if (c->flags_dirty) {
         int e = is_f64(c) ? pack_flags<double>(c) : pack_flags<float>(c);
         if (e) return e;
@@ -1141,6 +1152,7 @@ int lbffs_download(lbffs_ctx *c, int field, void *host, size_t bytes)
     if (bytes != field_host_bytes(c, field))
         return fail(LBFFS_E_INVALID, "download size " + std::to_string(bytes) + " != " +
                                          std::to_string(field_host_bytes(c, field)));
+    if (field == LBFFS_FIELD_QLINK && !c->qlink) return fail(LBFFS_E_STATE, "q-distances were never uploaded");
     if (field == LBFFS_FIELD_F) return dispatch<DownloadFOp>(c, host);
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(host, field_ptr(c, field), bytes, hipMemcpyDeviceToHost));
@@ -1287,6 +1299,8 @@ int lbffs_batch_step(lbffs_batch *b, int nsteps, const lbffs_step_params *params
         lbffs_ctx *c = b->ctx[d];
         if (!c->f_loaded) return fail(LBFFS_E_STATE, "lattice " + std::to_string(d) + ": populations never uploaded");
         if (ibm_active(c)) return fail(LBFFS_E_UNSUPPORTED, "moving-wall markers in a batch");
+        if (params[d].ibb && !c->qlink)
+            return fail(LBFFS_E_STATE, "lattice " + std::to_string(d) + ": ibb = 1 needs the q-distances");
         if (c->cur != cur0 || c->mcur != cur0)
             return fail(LBFFS_E_STATE, "lattice " + std::to_string(d) + " was stepped on its own: step parity differs");
         if (c->flags_dirty) {
@@ -1440,6 +1454,100 @@ int lbffs_geom_solid(int n, const float *v0, const float *v1, const float *v2, i
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemcpy(solid, d_solid, N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int lbffs_geom_qlinks(int q, int n, const float *v0, const float *v1, const float *v2, int nx, int ny, int nz,
+                      const uint8_t *solid, const uint8_t *neiSolid, float *qlink, int *misses)
+{
+    if (!solid || !neiSolid || !qlink || (n > 0 && (!v0 || !v1 || !v2))) return fail(LBFFS_E_INVALID, "null argument");
+    if (q != 19 && q != 27) return fail(LBFFS_E_INVALID, "q must be 19 or 27");
+    if (n < 0) return fail(LBFFS_E_INVALID, "negative triangle count");
+    if (int e = geom_dims_ok(nx, ny, nz)) return e;
+    const size_t N = (size_t)nx * ny * nz;
+    if ((double)q * N >= 4294967295.0) return fail(LBFFS_E_INVALID, "lattice too large for the q-distance array");
+    const int B = lbffs::kQBucket;
+    const int bx = (nx + B - 1) / B, by = (ny + B - 1) / B, bz = (nz + B - 1) / B;
+    const size_t nb = (size_t)bx * by * bz;
+    // triangles in FP64 and their buckets: every bucket the box grown by one cell overlaps
+    std::vector<lbffs::GeomTri64> tris((size_t)n);
+    std::vector<int> bstart(nb + 1, 0), btri;
+    std::vector<std::array<int, 6>> range((size_t)n);
+    for (int t = 0; t < n; t++) {
+        double a[3], b[3], c[3];
+        for (int ax = 0; ax < 3; ax++) {
+            a[ax] = v0[(size_t)ax * n + t];
+            b[ax] = v1[(size_t)ax * n + t];
+            c[ax] = v2[(size_t)ax * n + t];
+            tris[t].v0[ax] = a[ax];
+            tris[t].e1[ax] = b[ax] - a[ax];
+            tris[t].e2[ax] = c[ax] - a[ax];
+        }
+        const int dims[3] = {bx, by, bz};
+        bool ok = true;
+        for (int ax = 0; ax < 3; ax++) {
+            const double lo = std::min(a[ax], std::min(b[ax], c[ax])) - 1.0 - 1e-9;
+            const double hi = std::max(a[ax], std::max(b[ax], c[ax])) + 1.0 + 1e-9;
+            if (!(lo <= hi)) ok = false;  // NaN vertex
+            const double l = std::max(std::floor(lo / B), 0.0), h = std::min(std::floor(hi / B), (double)dims[ax] - 1);
+            if (l > h) ok = false;
+            range[t][2 * ax] = ok ? (int)l : 0;
+            range[t][2 * ax + 1] = ok ? (int)h : -1;
+        }
+        if (!ok) range[t] = {0, -1, 0, -1, 0, -1};
+        for (int kk = range[t][4]; kk <= range[t][5]; kk++)
+            for (int jj = range[t][2]; jj <= range[t][3]; jj++)
+                for (int ii = range[t][0]; ii <= range[t][1]; ii++) bstart[(size_t)ii + bx * ((size_t)jj + by * kk) + 1]++;
+    }
+    for (size_t b = 0; b < nb; b++) bstart[b + 1] += bstart[b];
+    if ((size_t)bstart[nb] > 0x7fffffffu) return fail(LBFFS_E_INVALID, "too many bucket entries");
+    btri.resize((size_t)bstart[nb]);
+    {
+        std::vector<int> fill(bstart.begin(), bstart.end() - 1);
+        for (int t = 0; t < n; t++)
+            for (int kk = range[t][4]; kk <= range[t][5]; kk++)
+                for (int jj = range[t][2]; jj <= range[t][3]; jj++)
+                    for (int ii = range[t][0]; ii <= range[t][1]; ii++)
+                        btri[(size_t)fill[(size_t)ii + bx * ((size_t)jj + by * kk)]++] = t;
+    }
+    std::vector<uint32_t> cells;
+    for (size_t ic = 0; ic < N; ic++)
+        if (neiSolid[ic] == 1 && solid[ic] == 0) cells.push_back((uint32_t)ic);
+    GeomScratch g;
+    lbffs::GeomTri64 *d_tris;
+    int *d_bstart, *d_btri;
+    uint32_t *d_cells, *d_miss;
+    uint8_t *d_solid;
+    float *d_q;
+    if (int e = g.alloc(&d_tris, tris.size())) return e;
+    if (int e = g.alloc(&d_bstart, bstart.size())) return e;
+    if (int e = g.alloc(&d_btri, btri.size())) return e;
+    if (int e = g.alloc(&d_cells, cells.size())) return e;
+    if (int e = g.alloc(&d_miss, 1)) return e;
+    if (int e = g.alloc(&d_solid, N)) return e;
+    if (int e = g.alloc(&d_q, (size_t)q * N)) return e;
+    if (n) HIP_TRY(hipMemcpy(d_tris, tris.data(), tris.size() * sizeof(lbffs::GeomTri64), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_bstart, bstart.data(), bstart.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!btri.empty()) HIP_TRY(hipMemcpy(d_btri, btri.data(), btri.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!cells.empty()) HIP_TRY(hipMemcpy(d_cells, cells.data(), cells.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_solid, solid, N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_q, qlink, (size_t)q * N * sizeof(float), hipMemcpyHostToDevice));
+    if (!cells.empty()) {
+        const unsigned grid = (unsigned)((cells.size() + 255) / 256);
+        if (q == 27)
+            hipLaunchKernelGGL(lbffs::geom_qlinks_kernel<27>, dim3(grid), dim3(256), 0, nullptr, d_cells,
+                               (uint32_t)cells.size(), d_tris, d_bstart, d_btri, bx, by, bz, nx, ny, nz, d_solid, d_q,
+                               d_miss);
+        else
+            hipLaunchKernelGGL(lbffs::geom_qlinks_kernel<19>, dim3(grid), dim3(256), 0, nullptr, d_cells,
+                               (uint32_t)cells.size(), d_tris, d_bstart, d_btri, bx, by, bz, nx, ny, nz, d_solid, d_q,
+                               d_miss);
+        HIP_TRY(hipGetLastError());
+    }
+    uint32_t miss = 0;
+    HIP_TRY(hipMemcpy(&miss, d_miss, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(qlink, d_q, (size_t)q * N * sizeof(float), hipMemcpyDeviceToHost));
+    if (misses) *misses = (int)miss;
     return 0;
 }
 

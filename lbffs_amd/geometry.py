@@ -159,6 +159,22 @@ def set_sdf(patch: Patch, sdf: np.ndarray, nei: np.ndarray, nx: int, ny: int, nz
     return sdf
 
 
+def qlinks(stl: STL, q: int, nx: int, ny: int, nz: int, solid, nei, init: float = 0.5):
+    """q-distances of the solid links from the surface in FP64 (lbffs_geom_qlinks), for the
+    interpolated bounce-back (Lattice field "qlink", step parameter ibb=1).  Returns the
+    q*N float32 array (init where a link has no solid neighbour or meets no triangle) and
+    the number of solid links that met no triangle."""
+    v = [np.ascontiguousarray(x, np.float32) for x in stl.triangles()]
+    s = np.ascontiguousarray(solid, np.uint8)
+    n8 = np.ascontiguousarray(nei, np.uint8)
+    assert s.size == n8.size == nx * ny * nz
+    ql = np.full(q * nx * ny * nz, init, np.float32)
+    miss = C.c_int()
+    _check(lib().lbffs_geom_qlinks(q, v[0].shape[1], *[_ptr(x) for x in v], nx, ny, nz, _ptr(s), _ptr(n8), _ptr(ql),
+                                   C.byref(miss)), "lbffs_geom_qlinks")
+    return ql, miss.value
+
+
 def sdf_near_bwalls(sdf: np.ndarray, b1, b2, b3, nx: int, ny: int, nz: int, dr: float = BWALL_DR) -> np.ndarray:
     """walls.hpp:372-469 setSDFnearBwalls (host); updates and returns sdf."""
     assert sdf.dtype == np.float32 and sdf.flags.c_contiguous and sdf.size == nx * ny * nz

@@ -145,3 +145,69 @@ int orc_geom_sdf(int n, const float *c, const float *nrm, float sdfIni, float dr
     free(near);
     return 0;
 }
+
+/* q-distances of the solid links (the engine's lbffs_geom_qlinks; the reference has no
+ * counterpart in use): for each cell with neiSolid == 1 and solid == 0 and each direction
+ * whose upwind cell (periodic wrap, walls.hpp:187-216) is solid, the smallest t in [0, 1]
+ * at which x - t c meets a triangle, by the Moller-Trumbore test in FP64 over every
+ * triangle.  Links without a hit keep their value and are counted. */
+static double link_hit(const double *v0, const double *e1, const double *e2, const double *x, const double *c)
+{
+    const double dir[3] = {-c[0], -c[1], -c[2]};
+    const double p[3] = {dir[1] * e2[2] - dir[2] * e2[1], dir[2] * e2[0] - dir[0] * e2[2], dir[0] * e2[1] - dir[1] * e2[0]};
+    const double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+    if (fabs(det) < 1e-14) return 2.0;
+    const double inv = 1.0 / det;
+    const double sv[3] = {x[0] - v0[0], x[1] - v0[1], x[2] - v0[2]};
+    const double u = (sv[0] * p[0] + sv[1] * p[1] + sv[2] * p[2]) * inv;
+    if (u < -1e-12 || u > 1.0 + 1e-12) return 2.0;
+    const double qv[3] = {sv[1] * e1[2] - sv[2] * e1[1], sv[2] * e1[0] - sv[0] * e1[2], sv[0] * e1[1] - sv[1] * e1[0]};
+    const double v = (dir[0] * qv[0] + dir[1] * qv[1] + dir[2] * qv[2]) * inv;
+    if (v < -1e-12 || u + v > 1.0 + 1e-12) return 2.0;
+    const double t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * inv;
+    return (t >= 0.0 && t <= 1.0) ? t : 2.0;
+}
+
+int orc_geom_qlinks(int Q, int n, const float *v0, const float *v1, const float *v2, int nx, int ny, int nz,
+                    const unsigned char *solid, const unsigned char *neiSolid, float *qlink, int *misses)
+{
+    const int *cx = Q == 27 ? ORC27_CX : ORC_CX, *cy = Q == 27 ? ORC27_CY : ORC_CY, *cz = Q == 27 ? ORC27_CZ : ORC_CZ;
+    const long N = (long)nx * ny * nz;
+    double *T = (double *)malloc(sizeof(double) * 9 * (n > 0 ? n : 1));
+    if (!T) return -1;
+    for (int t = 0; t < n; t++)
+        for (int a = 0; a < 3; a++) {
+            const double p0 = v0[(long)a * n + t], p1 = v1[(long)a * n + t], p2 = v2[(long)a * n + t];
+            T[9 * t + a] = p0;
+            T[9 * t + 3 + a] = p1 - p0;
+            T[9 * t + 6 + a] = p2 - p0;
+        }
+    int miss = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : miss)
+    for (long ic = 0; ic < N; ic++) {
+        if (neiSolid[ic] != 1 || solid[ic] != 0) continue;
+        const int i = (int)(ic % ((long)nx * ny) % nx), j = (int)(ic % ((long)nx * ny) / nx),
+                  k = (int)(ic / ((long)nx * ny));
+        const double x[3] = {(double)i, (double)j, (double)k};
+        for (int d = 1; d < Q; d++) {
+            int si = i - cx[d], sj = j - cy[d], sk = k - cz[d];
+            si = si < 0 ? nx - 1 : (si >= nx ? 0 : si);
+            sj = sj < 0 ? ny - 1 : (sj >= ny ? 0 : sj);
+            sk = sk < 0 ? nz - 1 : (sk >= nz ? 0 : sk);
+            if (solid[orc_index(si, sj, sk, nx, ny)] != 1) continue;
+            const double c[3] = {(double)cx[d], (double)cy[d], (double)cz[d]};
+            double best = 2.0;
+            for (int t = 0; t < n; t++) {
+                const double h = link_hit(&T[9 * t], &T[9 * t + 3], &T[9 * t + 6], x, c);
+                best = h < best ? h : best;
+            }
+            if (best <= 1.0)
+                qlink[(long)d * N + ic] = (float)best;
+            else
+                miss++;
+        }
+    }
+    free(T);
+    if (misses) *misses = miss;
+    return 0;
+}

@@ -23,6 +23,8 @@ typedef struct {
     int damping;   /* Smagorinsky wall damping on/off */
     float dpdx, LES, omegaB, spzWidth, Cs;
     float clim;    /* D3Q27 cumulant limiter (input.hpp default 0.01) */
+    int ibb;       /* 1: interpolated bounce-back with the precomputed q-distances below */
+    const void *qf; /* q-distances, Q*N reals of the step's precision (float or double), q-major */
 } orc_params;
 
 extern const int ORC_CX[19], ORC_CY[19], ORC_CZ[19];
@@ -100,6 +102,8 @@ int orc_geom_solid(int n, const float *v0, const float *v1, const float *v2, int
                    unsigned char *solid);
 int orc_geom_sdf(int n, const float *c, const float *nrm, float sdfIni, float dr, float p, int nx, int ny, int nz,
                  const unsigned char *neiSolid, float *sdf);
+int orc_geom_qlinks(int Q, int n, const float *v0, const float *v1, const float *v2, int nx, int ny, int nz,
+                    const unsigned char *solid, const unsigned char *neiSolid, float *qlink, int *misses);
 
 /* output step (output_oracle.c): cal_rhoUVW, D3Q19_opencl/D3Q19.hpp:647-666 */
 void orc_cal_rhouvw(int Q, long N, const float *f, float dpdx, float *rho, float *u, float *v, float *w);

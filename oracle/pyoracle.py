@@ -28,7 +28,8 @@ COLL_RR, COLL_BGK = 0, 1
 class OrcParams(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("order", C.c_int), ("sgs", C.c_int),
                 ("collision", C.c_int), ("damping", C.c_int), ("dpdx", C.c_float), ("LES", C.c_float),
-                ("omegaB", C.c_float), ("spzWidth", C.c_float), ("Cs", C.c_float), ("clim", C.c_float)]
+                ("omegaB", C.c_float), ("spzWidth", C.c_float), ("Cs", C.c_float), ("clim", C.c_float),
+                ("ibb", C.c_int), ("qf", C.c_void_p)]
 
 
 _orc = None
@@ -84,7 +85,21 @@ def _bind(L):
     L.orc_geom_solid.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
     L.orc_geom_sdf.argtypes = [C.c_int, vp, vp, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, vp, vp]
     L.orc_cal_rhouvw.argtypes = [C.c_int, C.c_long, vp, C.c_float, vp, vp, vp, vp]
+    L.orc_geom_qlinks.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp,
+                                  C.POINTER(C.c_int)]
     return L
+
+
+def geom_qlinks(q, v0, v1, v2, nx, ny, nz, solid, nei, init=0.5):
+    """FP64 ray-triangle q-distances (brute force over the triangles): (qlink, misses)."""
+    v = [np.ascontiguousarray(x, np.float32) for x in (v0, v1, v2)]
+    s = np.ascontiguousarray(solid, np.uint8)
+    n8 = np.ascontiguousarray(nei, np.uint8)
+    ql = np.full(q * nx * ny * nz, init, np.float32)
+    miss = C.c_int()
+    rc = orc().orc_geom_qlinks(q, v[0].shape[1], *[_p(x) for x in v], nx, ny, nz, _p(s), _p(n8), _p(ql), C.byref(miss))
+    assert rc == 0
+    return ql, miss.value
 
 
 def cal_rhouvw(q, f, dpdx=0.0):
@@ -155,8 +170,11 @@ def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=
     P = OrcParams(nx=case.nx, ny=case.ny, nz=case.nz, order=order, sgs=case.sgs if sgs is None else sgs,
                   collision=collision, damping=int(p.get("damping", 1)), dpdx=p.get("dpdx", 0.0),
                   LES=p.get("LES", 0.0), omegaB=p.get("omegaB", 0.1), spzWidth=p.get("spzWidth", 0.1),
-                  Cs=p.get("Cs", 0.1), clim=p.get("clim", 0.01))
+                  Cs=p.get("Cs", 0.1), clim=p.get("clim", 0.01), ibb=int(p.get("ibb", 0)))
     sides = {k: np.ascontiguousarray(a[k], real) for k in ("sdf", "u0", "v0", "w0", "omega")}
+    if P.ibb:
+        qf = np.ascontiguousarray(a["qlink"], real)
+        P.qf = qf.ctypes.data
     ints = {k: np.ascontiguousarray(a[k], np.int32) for k in ("boundary", "boundary1", "boundary2", "boundary3")}
     u8 = {k: np.ascontiguousarray(a[k], np.uint8) for k in ("solid", "neiSolid")}
     f = st["f"]

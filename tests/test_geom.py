@@ -344,3 +344,83 @@ def test_sdf_near_bwalls_skips_the_opposite_face():
     geo.sdf_near_bwalls(sdf, b1, z, z, nx, ny, nz, dr=3.0)
     # from i=0: layers i < nx-1 -> 0, 1; from i=nx-1: cells nx-1-d > 0 -> 2 (d=0), 1 (d=1)
     assert sdf.reshape(ny, nx)[0].tolist() == [0.5, 1.5, 0.5]
+
+
+# ---- q-distances from the surface (FP64) ---------------------------------------------
+
+def uv_sphere(c, R, nu=96, nv=48):
+    """Closed triangle mesh of a sphere (latitude-longitude), outward winding."""
+    cx, cy, cz = c
+    th = np.linspace(0.0, np.pi, nv + 1)
+    ph = np.linspace(0.0, 2 * np.pi, nu + 1)
+    P = lambda a, b: (cx + R * np.sin(th[a]) * np.cos(ph[b]), cy + R * np.sin(th[a]) * np.sin(ph[b]),
+                      cz + R * np.cos(th[a]))
+    tris = []
+    for a in range(nv):
+        for b in range(nu):
+            p00, p01, p10, p11 = P(a, b), P(a, b + 1), P(a + 1, b), P(a + 1, b + 1)
+            if a > 0:
+                tris.append((p00, p10, p01))
+            if a < nv - 1:
+                tris.append((p01, p10, p11))
+    return _mesh(tris)
+
+
+def test_oracle_qlinks_plane_is_exact():
+    # plane y = 5.3 across the lattice; fluid row j = 5 next to the solid rows j >= 6
+    nx, ny, nz = 8, 10, 6
+    y0 = 5.3
+    v0, v1, v2 = _mesh([((-3, y0, -3), (12, y0, -3), (12, y0, 10)), ((-3, y0, -3), (12, y0, 10), (-3, y0, 10))])
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    solid[:, 6:, :] = 1
+    nei = np.zeros((nz, ny, nx), np.uint8)
+    nei[:, 5, :] = 1
+    ql, miss = po.geom_qlinks(19, v0, v1, v2, nx, ny, nz, solid.reshape(-1), nei.reshape(-1))
+    assert miss == 0
+    ql = ql.reshape(19, nz, ny, nx)
+    cy = np.array(po.ORC19_C[1])
+    want = float(np.float32(y0)) - 5.0  # the link x - t c_q reaches the plane at t = y0 - 5
+    for d in range(1, 19):
+        if cy[d] == -1:   # upwind x - c_d is the solid row above
+            assert np.allclose(ql[d, :, 5, :], want, rtol=0, atol=1e-7), d
+        else:
+            assert np.all(ql[d] == 0.5), d
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_oracle_qlinks_sphere_analytic(q):
+    nx = ny = nz = 22
+    c, R = (10.3, 10.6, 10.2), 6.4
+    v0, v1, v2 = uv_sphere(c, R)
+    # solid cells inside the sphere itself (the ray-parity voxeliser of the reference flips
+    # whole row segments where a row grazes a shared edge of this mesh)
+    k3, j3, i3 = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r = np.sqrt((i3 - c[0]) ** 2 + (j3 - c[1]) ** 2 + (k3 - c[2]) ** 2).reshape(-1)
+    solid = (r < R).astype(np.uint8)
+    nei = cases.neisolid(q, nx, ny, nz, solid)
+    ql, miss = po.geom_qlinks(q, v0, v1, v2, nx, ny, nz, solid, nei)
+    # a solid cell closer to the sphere than the chords (0.014 cells) may lie outside the
+    # mesh: its links meet no triangle
+    assert miss <= 10
+    C3 = [np.array(a) for a in (po.ORC27_C if q == 27 else po.ORC19_C)]
+    ql = ql.reshape(q, -1)
+    ic = np.nonzero(nei)[0]
+    i, j, k = ic % nx, ic // nx % ny, ic // (nx * ny)
+    checked = 0
+    for d in range(1, q):
+        dx, dy, dz = C3[0][d], C3[1][d], C3[2][d]
+        up = (i - dx) + nx * ((j - dy) + ny * (k - dz))
+        m = solid[up] == 1
+        # |p - t c|^2 = R^2 with p = x - centre: the smaller root
+        px, py, pz = i[m] - c[0], j[m] - c[1], k[m] - c[2]
+        a = dx * dx + dy * dy + dz * dz
+        b = -(px * dx + py * dy + pz * dz)
+        cc = px * px + py * py + pz * pz - R * R
+        t = (-b - np.sqrt(np.maximum(b * b - a * cc, 0))) / a
+        # the mesh's facets lie inside the sphere by up to ~0.01 cells; a link that meets the
+        # surface at grazing incidence turns that into a larger change of t
+        hit = ql[d, ic[m]] != 0.5
+        err = np.abs(ql[d, ic[m]][hit] - t[hit])
+        assert err.max() < 0.1 and err.mean() < 0.01, (d, err.max(), err.mean())
+        checked += int(hit.sum())
+    assert checked > 500

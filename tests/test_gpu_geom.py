@@ -3,7 +3,8 @@
 Bar: the solid flags are bit-identical to the oracle's restatement of setSolid (and, on
 the integer meshes, to exact arithmetic); the wall distance of setSDF agrees to FP32
 rounding -- the reference takes r^-p from the host's powf, the kernel rounds it once from
-FP64, so a few terms may differ in their last bit.
+FP64, so a few terms may differ in their last bit; the FP64 ray-triangle q-distances are
+bit-identical to the oracle's brute force.
 """
 import time
 
@@ -142,3 +143,54 @@ def test_preprocess_backstep_les():
     assert np.array_equal(g["neiSolid"], nei)
     assert np.array_equal(g["sdf"] == 10000.0, sdf == 10000.0)
     assert np.abs(g["sdf"] - sdf).max() <= 2e-5 * max(1.0, float(np.abs(sdf[sdf != 10000.0]).max()))
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_qlinks_bitwise(q):
+    # FP64 ray-triangle q-distances: GPU kernel (bucketed triangles) against the oracle's
+    # brute force over every triangle, bit for bit
+    from test_geom import uv_sphere
+    nx = ny = nz = 24
+    c, R = (11.3, 11.6, 12.2), 7.4
+    v0, v1, v2 = uv_sphere(c, R)
+    solid = geo.solid_from_triangles(v0, v1, v2, nx, ny, nz)
+    nei = cases.neisolid(q, nx, ny, nz, solid)
+    t0 = time.perf_counter()
+    g, gm = geo.qlinks(geo.STL([geo.Patch("s", v0, v1, v2)]), q, nx, ny, nz, solid, nei)
+    t1 = time.perf_counter()
+    o, om = po.geom_qlinks(q, v0, v1, v2, nx, ny, nz, solid, nei)
+    print(f"sphere q{q}: {int((g != 0.5).sum())} links, GPU {1e3 * (t1 - t0):.1f} ms, misses {gm}", flush=True)
+    assert gm == om
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    assert (g != 0.5).sum() > 500
+
+
+def test_qlinks_cylinder_stl():
+    # the reference's cylinder STL (its side surface only: an open tube, as the 2-D case
+    # needs), raised by 12 cells so that its ends lie inside the lattice: q-distances on
+    # the GPU equal the oracle's, and every solid link through the side meets the mesh --
+    # the links that miss are those through the open ends (fluid layers k = 2 and k = 22)
+    nx, ny, nz = 240, 240, 26
+    s = geo.read_stl(CYL_STL, L_CYL)
+    for p in s.patches:
+        for v in (p.v0, p.v1, p.v2):
+            v[2] += np.float32(12.0)
+    solid = geo.set_solid(s, nx, ny, nz)
+    nei = cases.neisolid(27, nx, ny, nz, solid)
+    g, gm = geo.qlinks(s, 27, nx, ny, nz, solid, nei, init=-1.0)
+    o, om = po.geom_qlinks(27, *s.triangles(), nx, ny, nz, solid, nei, init=-1.0)
+    n_links = int((g >= 0).sum())
+    print(f"cylinder STL q27: {n_links} links, misses {gm}", flush=True)
+    assert gm == om and np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    assert n_links > 5000
+    # misses only through the open ends: recount the solid links off those layers
+    g = g.reshape(27, nz, ny, nx)
+    nei3 = nei.reshape(nz, ny, nx).astype(bool)
+    sol3 = solid.reshape(nz, ny, nx)
+    cx, cy, cz = (np.array(a) for a in po.ORC27_C)
+    for d in range(1, 27):
+        up = np.roll(np.roll(np.roll(sol3, cz[d], axis=0), cy[d], axis=1), cx[d], axis=2) == 1
+        side = nei3 & up
+        side[:3] = False
+        side[22:] = False
+        assert (g[d][side] >= 0).all(), d

@@ -47,6 +47,27 @@ def test_cylinder_channel_vs_oracle():
     assert abs(h["reduce"][0] - fw_h[0]) <= 1e-6 * max(1.0, abs(fw_h[0]))
 
 
+@pytest.mark.parametrize("sgs", [lb.SGS_WALE, lb.SGS_SMAGORINSKY])
+def test_cylinder_channel_ibb_vs_oracle(sgs):
+    # Filippova-Hanel interpolated bounce-back with analytic q-distances (ibb = 1)
+    c = cases.cylinder_channel(nx=96, ny=48, nz=8, D=12, cx=30, cy=24, Re=60.0, spz_cells=8, sgs=sgs, ibb=True)
+    h, o, _ = parity(c, 150, label=f"cylinder ibb sgs={sgs}")
+    fw_h = np.array([h["Fwx"].astype(np.float64).sum(), h["Fwy"].astype(np.float64).sum()])
+    fw_o = np.array([o["Fwx"].astype(np.float64).sum(), o["Fwy"].astype(np.float64).sum()])
+    assert rel_l2(fw_h, fw_o) <= 1e-4, (fw_h, fw_o)
+    # the interpolated walls move the drag away from the bounce-back run's
+    b = run_hip(cases.cylinder_channel(nx=96, ny=48, nz=8, D=12, cx=30, cy=24, Re=60.0, spz_cells=8, sgs=sgs), 150)
+    assert abs(b["Fwx"].astype(np.float64).sum() - fw_h[0]) > 1e-3 * abs(fw_h[0])
+
+
+def test_ibb_without_qlinks_is_refused():
+    c = cases.cylinder_channel(nx=32, ny=16, nz=2, D=6, cx=10, cy=8, Re=40.0, spz_cells=4)
+    with lb.Lattice(c.nx, c.ny, c.nz, sgs=c.sgs, omega0=c.omega0) as lat:
+        c.upload_to(lat)
+        with pytest.raises(lb.LbffsError, match="q-distances"):
+            lat.step(1, ibb=1, **c.params)
+
+
 def test_backstep_smagorinsky_vs_oracle():
     c = cases.backward_step(nx=128, ny=32, nz=8, H=12, step_len=24, spz_cells=16)
     parity(c, 100, label="backstep")

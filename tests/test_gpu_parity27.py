@@ -51,6 +51,15 @@ def test_cylinder27_ibb_vs_oracle(sdf_from_geometry):
     assert rel_l2(fw_h, fw_o) <= 1e-4, (fw_h, fw_o)
 
 
+def test_cylinder27_qlinks_vs_oracle():
+    # interpolated bounce-back with analytic q-distances instead of the distance ratio
+    c = cases.cylinder_channel(nx=64, ny=32, nz=6, D=10, cx=20, cy=16, Re=60.0, spz_cells=6, q=27, ibb=True)
+    h, o, _ = parity(c, 100, label="cylinder q27 qlinks")
+    fw_h = np.array([h["Fwx"].astype(np.float64).sum(), h["Fwy"].astype(np.float64).sum()])
+    fw_o = np.array([o["Fwx"].astype(np.float64).sum(), o["Fwy"].astype(np.float64).sum()])
+    assert rel_l2(fw_h, fw_o) <= 1e-4, (fw_h, fw_o)
+
+
 def test_mixed_faces27_vs_oracle():
     # FixedDensity outlet, Symmetry (mirrored links) and WallFunction faces, FixedVelocity inlet
     codes = {f: ("Cyclic", "Cyclic", "Cyclic") for f in cases.FACES}
