@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--only", default="", help="comma list of storage forms to run (f32,f16,bf16)")
+    ap.add_argument("--forces", type=int, default=0,
+                    help="cylinder: also run this many steps per storage form (and FP64) and report Cd, Cl, St")
     return ap.parse_args()
 
 
@@ -51,7 +53,9 @@ def build_case(name):
     if name == "backstep":
         return cases.backward_step()
     if name == "cylinder":
-        return cases.cylinder_channel()
+        # channel past a cylinder: 400 x 160 x 16, D = 40 at (100, 80), inlet 0.05 with 1e-3
+        # noise, outlet + 20-cell sponge, Re = 100, interpolated bounce-back with analytic q
+        return cases.cylinder_channel(ibb=True)
     if name == "box27":
         # D3Q27 cumulant step on a periodic 250^3 box (15.6M cells), no sub-grid model
         return cases.periodic_box(250, 250, 250, tau=0.8, sgs=lb.SGS_NONE, q=27)
@@ -189,6 +193,56 @@ def ensemble_main(a):
     print(json.dumps(out), flush=True)
 
 
+def force_history(case, storage, nsteps, every=50):
+    """Drag and lift coefficients of the cylinder every `every` steps, from the FP64 device
+    sums of the solid-link momentum: C = F / (0.5 rho u_in^2 D nz)."""
+    lat = lb.Lattice(case.nx, case.ny, case.nz, storage=storage, sgs=case.sgs, omega0=case.omega0)
+    case.upload_to(lat)
+    D, u = case.meta["D"], case.meta["u_in"]
+    norm = 0.5 * u * u * D * case.nz
+    cd, cl = [], []
+    for _ in range(nsteps // every):
+        lat.step(every, **case.params)
+        s = lat.reduce()
+        cd.append(s[0] / norm)
+        cl.append(s[1] / norm)
+    lat.close()
+    return np.array(cd), np.array(cl)
+
+
+def shedding(cd, cl, every, D, u):
+    """Mean Cd, rms Cl and the Strouhal number of the second half of the history (the
+    dominant frequency of Cl, by FFT with a parabolic peak refinement)."""
+    h = len(cl) // 2
+    c = cl[h:] - cl[h:].mean()
+    spec = np.abs(np.fft.rfft(c * np.hanning(len(c))))
+    k = int(np.argmax(spec[1:])) + 1
+    if 1 <= k < len(spec) - 1:
+        a, b, g = spec[k - 1], spec[k], spec[k + 1]
+        k = k + 0.5 * (a - g) / (a - 2 * b + g)
+    f = k / (len(c) * every)
+    return {"Cd": float(cd[h:].mean()), "Cl_rms": float(np.sqrt(np.mean(c * c))), "St": float(f * D / u)}
+
+
+def forces_report(case, nsteps, forms):
+    every = 50
+    D, u = case.meta["D"], case.meta["u_in"]
+    out = {}
+    for name, st in list(forms.items()) + [("f64", lb.STORE_F64)]:
+        log(f"[bench] forces {name}: {nsteps} steps")
+        cd, cl = force_history(case, st, nsteps, every)
+        out[name] = shedding(cd, cl, every, D, u)
+    ref = out["f64"]
+    for name in forms:
+        r = out[name]
+        r["Cd_rel_err_vs_f64"] = abs(r["Cd"] - ref["Cd"]) / abs(ref["Cd"])
+        r["St_rel_err_vs_f64"] = abs(r["St"] - ref["St"]) / abs(ref["St"]) if ref["St"] else None
+        r["Cl_rms_rel_err_vs_f64"] = abs(r["Cl_rms"] - ref["Cl_rms"]) / max(abs(ref["Cl_rms"]), 1e-12)
+    out["steps"] = nsteps
+    out["sample_every"] = every
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -252,7 +306,7 @@ def main():
         "dtype": "f32" if main_form == "f32" else ("f16-storage/f32-math" if main_form == "f16" else "bf16-storage/f32-math"),
         "data": "synthetic",
         "config": {"workload": {"backstep": "backward-facing step D3Q19 Smagorinsky LES Re_H=5500",
-                                "cylinder": "channel past cylinder D3Q19 Re=100",
+                                "cylinder": "channel past cylinder D3Q19 Re=100, interpolated bounce-back with analytic q",
                                 "cavity": "lid-driven cavity D3Q19 Re=100",
                                 "box27": "periodic box D3Q27 cumulant, no SGS"}[a.workload],
                    "lattice": [case.nx, case.ny, case.nz], "cells": N, "fluid_cells": fluid,
@@ -274,6 +328,8 @@ def main():
     }
     if "f16" in res and "f32" in res:
         out["forms"]["f16_over_f32"] = round(res["f16"]["mlups"] / res["f32"]["mlups"], 3)
+    if a.forces and a.workload == "cylinder":
+        out["forces"] = forces_report(case, a.forces, {k: forms[k] for k in run})
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (oracle on host cores)")
         out["cpu_baseline"] = cpu_baseline(case, 3 if N > 10_000_000 else 20)

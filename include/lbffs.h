@@ -104,8 +104,16 @@ typedef struct {
     int collision; /* LBFFS_COLL_* */
     float omega0;  /* 1/(3 nu + 1/2): cells whose omega equals it skip the omega read */
     int block;     /* work-group size; 0 = engine default */
-    int reserved[8];
+    int rounding;  /* 16-bit storage forms: LBFFS_ROUND_STOCHASTIC (default) or LBFFS_ROUND_NEAREST */
+    int reserved[7];
 } lbffs_desc;
+
+/* Rounding of the 16-bit storage forms (no counterpart in the reference, which stores
+ * FP32).  Round-to-nearest removes every perturbation below half a unit in the last place
+ * at each step, which can hold a flow on a symmetric, non-physical branch (a cylinder wake
+ * at Re 100 never sheds); stochastic rounding (gfx950 v_cvt_sr_*) is unbiased, with the
+ * random bits hashed from the cell's moments: deterministic for a given state. */
+enum { LBFFS_ROUND_STOCHASTIC = 0, LBFFS_ROUND_NEAREST = 1 };
 
 /* Scalars of one k_streamingCollision launch (D3Q19 ns.cpp:528-535, K19:3541-3545;
  * D3Q27 ns.cpp:555-577, K27:2725-2731). */

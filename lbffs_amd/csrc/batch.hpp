@@ -34,11 +34,13 @@ __global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_e
         if (icu >= a.N) continue;
         do {
 #define CELL_RETURN continue
+#define CELL_IBB (a.ibb != 0)
             if constexpr (Q == 27) {
 #include "d3q27_cell.inc"
             } else {
 #include "d3q19_cell.inc"
             }
+#undef CELL_IBB
 #undef CELL_RETURN
         } while (0);
     }

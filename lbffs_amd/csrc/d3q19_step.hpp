@@ -57,6 +57,7 @@ struct StepArgs {
     Real clim;  // cumulant limiter (D3Q27)
     const Real *qlink;  // q-distances of the solid links (Q*N, q-major), read when ibb
     int ibb;            // interpolated bounce-back with qlink instead of the reference's walls
+    int sround;         // 16-bit forms: stochastic rounding of the stored deviations
     int damping;
     int mac_all;      // write rho/u/v/w for every fluid cell
     int aux_all;      // write tauSGS / Fw / G for every fluid cell
@@ -625,14 +626,19 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
     }
 }
 
-template <typename S, typename Real, int SGS>
+// IBB selects the interpolated bounce-back at compile time: its loop raises the register
+// count of the whole kernel from 63 to 72 VGPRs (7 instead of 8 waves per SIMD), which the
+// 16-bit forms, bound by instruction issue, feel.
+template <typename S, typename Real, int SGS, bool IBB = false>
 __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
 {
     const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     const uint32_t icu = blk * blockDim.x + threadIdx.x;
     if (icu >= a.N) return;
 #define CELL_RETURN return
+#define CELL_IBB IBB
 #include "d3q19_cell.inc"
+#undef CELL_IBB
 #undef CELL_RETURN
 }
 

@@ -447,6 +447,7 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
     a.Cs = (Real)p->Cs;
     a.clim = (Real)p->clim;
     a.ibb = p->ibb ? 1 : 0;
+    a.sround = (c->store_size == 2 && c->d.rounding == LBFFS_ROUND_STOCHASTIC) ? 1 : 0;
     a.qlink = (const Real *)c->qlink;
     a.omega0 = (Real)c->d.omega0;
     a.damping = p->damping;
@@ -461,10 +462,14 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
 template <typename S, typename Real, int Q, int SGS>
 void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream_t st)
 {
-    if constexpr (Q == 27)
+    if constexpr (Q == 27) {
         hipLaunchKernelGGL((step27_kernel<S, Real, SGS>), dim3(grid), dim3(block), 0, st, a);
-    else
-        hipLaunchKernelGGL((step19_kernel<S, Real, SGS>), dim3(grid), dim3(block), 0, st, a);
+    } else {
+        if (a.ibb)
+            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true>), dim3(grid), dim3(block), 0, st, a);
+        else
+            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false>), dim3(grid), dim3(block), 0, st, a);
+    }
 }
 
 template <typename Real>
@@ -874,6 +879,8 @@ int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out)
     if (N >= (1L << 31) / 1 || (long)desc->q * N < 0) return fail(LBFFS_E_INVALID, "cell count must be below 2^31");
     if (desc->storage < 0 || desc->storage > 3) return fail(LBFFS_E_INVALID, "storage");
     if (desc->sgs < 0 || desc->sgs > 2) return fail(LBFFS_E_INVALID, "sgs");
+    if (desc->rounding != LBFFS_ROUND_STOCHASTIC && desc->rounding != LBFFS_ROUND_NEAREST)
+        return fail(LBFFS_E_INVALID, "rounding");
     if (desc->q == 19 && desc->collision != LBFFS_COLL_RR)
         return fail(LBFFS_E_UNSUPPORTED, "D3Q19 runs the recursive-regularized collision (K19:3637)");
     if (desc->q == 27 && desc->collision != LBFFS_COLL_CUMULANT)

@@ -108,7 +108,30 @@ __device__ inline uint16_t float_to_bf16(float x)
     return (uint16_t)(u >> 16);
 }
 
-// Store<S,Real>: load/store of population q with weight w as compute type Real.
+// Stochastic rounding of the 16-bit forms.  Rounding each stored deviation to the nearest
+// 16-bit value removes perturbations below half a unit in the last place every step, and a
+// flow held symmetric by that rounding never leaves it (a cylinder wake at Re 100 does not
+// shed); rounding up with the probability of the remainder keeps the stored value unbiased.
+// The random bits come from a hash of the cell's moments of this step and its index -- a
+// deterministic function of the state, the same in every launch shape, batch and graph
+// replay.
+__device__ inline uint32_t sr_hash(float a, float b, float c, float d, uint32_t ic)
+{
+    const uint32_t ub = __float_as_uint(b), uc = __float_as_uint(c), ud = __float_as_uint(d);
+    uint32_t h = __float_as_uint(a) ^ ((ub << 11) | (ub >> 21)) ^ ((uc << 19) | (uc >> 13)) ^ ((ud << 27) | (ud >> 5)) ^
+                 (ic * 0x9E3779B1u);
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 15;
+    return h;
+}
+
+// the bits of direction q: the cell's hash advanced by q golden-ratio steps, which spreads
+// the 19 (27) rounding thresholds of a cell evenly over [0, 1) -- one add per direction
+__device__ inline uint32_t sr_bits(uint32_t h, int q) { return h + (uint32_t)q * 0x9E3779B9u; }
+
+// Store<S,Real>: load/store of population q with weight w as compute type Real;
+// store_nt_sr(..., r) is the non-temporal store with stochastic rounding from the random
+// bits r (the 32-bit forms store exactly and ignore r).
 template <typename S, typename Real, int Q>
 struct Codec;
 
@@ -117,6 +140,7 @@ struct Codec<float, float, Q> {
     __device__ static inline float load(const float *p, int, float) { return *p; }
     __device__ static inline void store(float *p, float v, int, float) { *p = v; }
     __device__ static inline void store_nt(float *p, float v, int, float) { __builtin_nontemporal_store(v, p); }
+    __device__ static inline void store_nt_sr(float *p, float v, int q, float w, uint32_t) { store_nt(p, v, q, w); }
 };
 
 template <int Q>
@@ -124,6 +148,7 @@ struct Codec<double, double, Q> {
     __device__ static inline double load(const double *p, int, double) { return *p; }
     __device__ static inline void store(double *p, double v, int, double) { *p = v; }
     __device__ static inline void store_nt(double *p, double v, int, double) { __builtin_nontemporal_store(v, p); }
+    __device__ static inline void store_nt_sr(double *p, double v, int q, double w, uint32_t) { store_nt(p, v, q, w); }
 };
 
 template <int Q>
@@ -141,6 +166,16 @@ struct Codec<__half, float, Q> {
         __half h = __float2half_rn((v - w) * DevScale<Q>::scale(q));
         __builtin_nontemporal_store(__half_as_ushort(h), (unsigned short *)p);
     }
+    __device__ static inline void store_nt_sr(__half *p, float v, int q, float w, uint32_t r)
+    {
+        typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+        h2_t h = {0, 0};
+        h = __builtin_amdgcn_cvt_sr_f16_f32(h, (v - w) * DevScale<Q>::scale(q), r, false);
+        const _Float16 e = h[0];
+        unsigned short bits;
+        __builtin_memcpy(&bits, &e, 2);
+        __builtin_nontemporal_store(bits, (unsigned short *)p);
+    }
 };
 
 template <int Q>
@@ -156,6 +191,13 @@ struct Codec<bf16_t, float, Q> {
     __device__ static inline void store_nt(bf16_t *p, float v, int q, float w)
     {
         __builtin_nontemporal_store(float_to_bf16((v - w) * DevScale<Q>::scale(q)), (unsigned short *)p);
+    }
+    __device__ static inline void store_nt_sr(bf16_t *p, float v, int q, float w, uint32_t r)
+    {
+        typedef short b2_t __attribute__((ext_vector_type(2)));
+        b2_t h = {0, 0};
+        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, (v - w) * DevScale<Q>::scale(q), r, false);
+        __builtin_nontemporal_store((unsigned short)h[0], (unsigned short *)p);
     }
 };
 
