@@ -58,3 +58,109 @@ def test_fp16_stochastic_rounding_sheds_like_fp32():
     assert abs(res["f16_sr"]["Cd"] - res["f32"]["Cd"]) <= 0.01 * res["f32"]["Cd"]
     # round-to-nearest holds the wake symmetric: no shedding
     assert res["f16_rn"]["Cl_rms_late"] < 0.1 * res["f32"]["Cl_rms_late"]
+
+
+def _bracket16(d, bf16):
+    """The two 16-bit values next to each float32 d (equal where d is representable)."""
+    d = np.asarray(d, np.float32)
+    if not bf16:
+        rn = d.astype(np.float16)
+        up = rn.astype(np.float32) < d
+        dn = rn.astype(np.float32) > d
+        lo = np.where(up, rn, np.where(dn, np.nextafter(rn, np.float16(-np.inf)), rn)).astype(np.float32)
+        hi = np.where(up, np.nextafter(rn, np.float16(np.inf)), rn).astype(np.float32)
+        return lo, hi
+    b = d.view(np.uint32)
+    t = (b & np.uint32(0xFFFF0000)).view(np.float32)          # toward zero
+    away = ((b & np.uint32(0xFFFF0000)) + np.uint32(0x10000)).view(np.float32)
+    exact = t == d
+    lo = np.where(exact | (d > 0), t, away)
+    hi = np.where(exact | (d < 0), t, away)
+    return lo.astype(np.float32), hi.astype(np.float32)
+
+
+@pytest.mark.parametrize("q,storage", [(19, lb.STORE_F16DEV), (27, lb.STORE_F16DEV), (19, lb.STORE_BF16DEV)])
+def test_stochastic_rounding_is_unbiased(q, storage):
+    # a uniform stream on a periodic lattice: every interior cell computes the same new
+    # populations x_q, so the stored 16-bit deviations of one step are independent
+    # roundings of one value.  Each must be one of the two neighbours of x_q's deviation,
+    # and the share rounded up must equal x_q's position between them (the rounding is
+    # unbiased); with nearest rounding every cell stores the nearest value.
+    n = 32
+    c = cases.periodic_box(n, n, n, tau=0.9, ini=(0.0123, 0.0071, -0.0049), q=q)
+    coll = lb.COLL_CUMULANT if q == 27 else lb.COLL_RR
+    bf16 = storage == lb.STORE_BF16DEV
+    inner = np.zeros((n, n, n), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    inner = inner.reshape(-1)
+
+    def lattice(st, rnd=0):
+        return lb.Lattice(n, n, n, q=q, storage=st, sgs=c.sgs, collision=coll, omega0=c.omega0, rounding=rnd)
+
+    with lattice(storage) as lat:
+        c.upload_to(lat)
+        f0 = lat.download("f")                        # the 16-bit state, decoded
+    with lattice(lb.STORE_F32) as lat:
+        c.upload_to(lat)
+        lat.upload("f", f0)
+        lat.step(1, **c.params)
+        x = lat.download("f").reshape(q, -1)[:, inner]
+        u = lat.download("u")
+    assert abs(float(u[inner].mean())) > 1e-3
+    assert (x == x[:, :1]).all(), "the FP32 step is not uniform"
+    x = x[:, 0]
+    w = np.array([float(lb_w) for lb_w in _weights(q)], np.float32)
+    scale = np.array([_scale(q, d) for d in range(q)], np.float32)
+    d = (x - w) * scale
+    lo, hi = _bracket16(d, bf16)
+    dec_lo, dec_hi = w + lo / scale, w + hi / scale
+    for rnd in (lb.ROUND_STOCHASTIC, lb.ROUND_NEAREST):
+        with lattice(storage, rnd) as lat:
+            c.upload_to(lat)
+            lat.step(1, **c.params)
+            y = lat.download("f").reshape(q, -1)[:, inner]
+        m = y.shape[1]
+        for k in range(q):
+            is_lo, is_hi = y[k] == dec_lo[k], y[k] == dec_hi[k]
+            assert (is_lo | is_hi).all(), (rnd, k)
+            if rnd == lb.ROUND_NEAREST:
+                near = dec_hi[k] if abs(hi[k] - d[k]) < abs(d[k] - lo[k]) else dec_lo[k]
+                if abs(hi[k] - d[k]) != abs(d[k] - lo[k]):
+                    assert (y[k] == near).all(), k
+                continue
+            p = 0.0 if hi[k] == lo[k] else float((d[k] - lo[k]) / (hi[k] - lo[k]))
+            share = float(is_hi.mean()) if hi[k] != lo[k] else 0.0
+            sigma = np.sqrt(max(p * (1 - p), 1e-12) / m)
+            assert abs(share - p) < 6 * sigma + 1e-9, (k, share, p)
+        if rnd == lb.ROUND_STOCHASTIC:
+            print(f"q{q} storage {storage}: {m} cells, rounded-up shares vs positions ok", flush=True)
+
+
+def _weights(q):
+    if q == 19:
+        return [1 / 3] + [1 / 18] * 6 + [1 / 36] * 12
+    return [8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8
+
+
+def _scale(q, d):
+    if d == 0:
+        return 4.0
+    if d <= 6:
+        return 16.0
+    if q == 19 or d <= 18:
+        return 32.0 if q == 19 else 64.0
+    return 256.0
+
+
+def test_stochastic_rounding_ignores_the_launch_shape():
+    # the random bits come from the cell's own state and index, so the result does not
+    # depend on the work-group size or order
+    c = cases.cylinder_channel(nx=96, ny=48, nz=4, D=12, cx=30, cy=24, Re=150.0, q=19)
+    out = []
+    for block, tune in ((256, 0), (64, 2), (128, 0)):
+        with lb.Lattice(c.nx, c.ny, c.nz, storage=lb.STORE_F16DEV, sgs=c.sgs, omega0=c.omega0) as lat:
+            c.upload_to(lat)
+            lat.set_launch(block, tune)
+            lat.step(40, **c.params)
+            out.append(lat.download("f"))
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
