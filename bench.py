@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Throughput of the fused D3Q19 time step on one MI355X, in the reference's metric.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload backstep|cylinder|cavity|box27|ensemble]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload backstep|cylinder|cavity|box27|bluff27|ensemble]
 
 Default workload: backward-facing step, D3Q19, Smagorinsky LES (Cs=0.1, wall damping),
 1536 x 192 x 128 lattice (37.7M cells), FixedVelocity inlet with seeded 5 % Gaussian
@@ -39,13 +40,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="backstep", choices=["backstep", "cylinder", "cavity", "box27", "ensemble"])
+    ap.add_argument("--workload", default="backstep", choices=["backstep", "cylinder", "cavity", "box27", "bluff27", "ensemble"])
     ap.add_argument("--domains", type=int, default=512, help="ensemble: number of lattices")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--only", default="", help="comma list of storage forms to run (f32,f16,bf16)")
     ap.add_argument("--forces", type=int, default=0,
-                    help="cylinder: also run this many steps per storage form (and FP64) and report Cd, Cl, St")
+                    help="cylinder, bluff27: also run this many steps per storage form (and FP64) and report "
+                         "Cd, Cl, St")
     return ap.parse_args()
 
 
@@ -59,6 +61,10 @@ def build_case(name):
     if name == "box27":
         # D3Q27 cumulant step on a periodic 250^3 box (15.6M cells), no sub-grid model
         return cases.periodic_box(250, 250, 250, tau=0.8, sgs=lb.SGS_NONE, q=27)
+    if name == "bluff27":
+        # D3Q27 wind tunnel 1024 x 384 x 320 (126M cells) around a seeded procedural body of
+        # 199,200 triangles (D = 80), inlet 0.04, Re 300, WALE, FP64 q-distances from the mesh
+        return cases.bluff_body()
     return cases.cavity(64)
 
 
@@ -194,12 +200,15 @@ def ensemble_main(a):
 
 
 def force_history(case, storage, nsteps, every=50):
-    """Drag and lift coefficients of the cylinder every `every` steps, from the FP64 device
-    sums of the solid-link momentum: C = F / (0.5 rho u_in^2 D nz)."""
-    lat = lb.Lattice(case.nx, case.ny, case.nz, storage=storage, sgs=case.sgs, omega0=case.omega0)
+    """Drag and lift coefficients every `every` steps, from the FP64 device sums of the
+    solid-link momentum: C = F / (0.5 rho u_in^2 A), A = D nz for the cylinder, the
+    frontal area of the mesh for a bluff body."""
+    coll = lb.COLL_CUMULANT if case.q == 27 else lb.COLL_RR
+    lat = lb.Lattice(case.nx, case.ny, case.nz, q=case.q, storage=storage, sgs=case.sgs, collision=coll,
+                     omega0=case.omega0)
     case.upload_to(lat)
     D, u = case.meta["D"], case.meta["u_in"]
-    norm = 0.5 * u * u * D * case.nz
+    norm = 0.5 * u * u * case.meta.get("frontal_area", D * case.nz)
     cd, cl = [], []
     for _ in range(nsteps // every):
         lat.step(every, **case.params)
@@ -252,11 +261,14 @@ def main():
     if a.workload == "ensemble":
         return ensemble_main(a)
     log(f"[bench] building workload {a.workload}")
+    t_build = time.perf_counter()
     case = build_case(a.workload)
+    log(f"[bench] built {case.nx}x{case.ny}x{case.nz} in {time.perf_counter() - t_build:.1f} s")
     N = case.N
     fluid = int(N - int(case.arrays["solid"].sum()))
     forms = {"f32": lb.STORE_F32, "f16": lb.STORE_F16DEV, "bf16": lb.STORE_BF16DEV}
-    run = [f for f in (a.only.split(",") if a.only else ["f32", "f16"]) if f]
+    default_forms = ["f32", "bf16"] if a.workload == "bluff27" else ["f32", "f16"]
+    run = [f for f in (a.only.split(",") if a.only else default_forms) if f]
     res = {}
     vel = {}
     for name in run:
@@ -289,7 +301,8 @@ def main():
         probes[name] = round(mult * copy_bytes / (ms * 1e-3) / 1e9, 1)
     probes["copy"] = round(copy_Bps / 1e9, 1)
     log(f"[bench] HBM probes GB/s: {probes}")
-    main_form = "f32" if "f32" in res else run[0]
+    # the bluff body is quoted in its 16-bit form (BF16 deviations, 108 B per cell update)
+    main_form = "bf16" if a.workload == "bluff27" and "bf16" in res else ("f32" if "f32" in res else run[0])
     r = res[main_form]
     traffic = read_pmc() if a.workload == "backstep" and main_form == "f32" else None
     out = {
@@ -308,7 +321,9 @@ def main():
         "config": {"workload": {"backstep": "backward-facing step D3Q19 Smagorinsky LES Re_H=5500",
                                 "cylinder": "channel past cylinder D3Q19 Re=100, interpolated bounce-back with analytic q",
                                 "cavity": "lid-driven cavity D3Q19 Re=100",
-                                "box27": "periodic box D3Q27 cumulant, no SGS"}[a.workload],
+                                "box27": "periodic box D3Q27 cumulant, no SGS",
+                                "bluff27": "bluff body D3Q27 cumulant WALE Re=300, procedural STL (199,200 "
+                                           "triangles), FP64 q-distances, interpolated bounce-back"}[a.workload],
                    "lattice": [case.nx, case.ny, case.nz], "cells": N, "fluid_cells": fluid,
                    "collision": "cumulant" if case.q == 27 else "recursive-regularized",
                    "sgs": {0: "WALE", 1: "Smagorinsky", 2: "none"}[case.sgs],
@@ -328,11 +343,23 @@ def main():
     }
     if "f16" in res and "f32" in res:
         out["forms"]["f16_over_f32"] = round(res["f16"]["mlups"] / res["f32"]["mlups"], 3)
-    if a.forces and a.workload == "cylinder":
+    if "bf16" in res and "f32" in res:
+        out["forms"]["bf16_over_f32"] = round(res["bf16"]["mlups"] / res["f32"]["mlups"], 3)
+    if case.surface is not None or "qlink_misses" in case.meta:
+        out["config"]["qlink_misses"] = case.meta.get("qlink_misses")
+        out["config"]["triangles"] = case.meta.get("triangles")
+    if a.forces and a.workload in ("cylinder", "bluff27"):
         out["forces"] = forces_report(case, a.forces, {k: forms[k] for k in run})
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (oracle on host cores)")
-        out["cpu_baseline"] = cpu_baseline(case, 3 if N > 10_000_000 else 20)
+        if a.workload == "bluff27":
+            # the same wind tunnel scaled to 256 x 96 x 80 cells (body D = 20), q-distances
+            # as a host array for the oracle
+            small = cases.bluff_body(nx=256, ny=96, nz=80, D=20.0, n_lat=80, n_lon=128, spz_cells=16,
+                                     qlinks_on_device=False)
+            out["cpu_baseline"] = cpu_baseline(small, 3)
+        else:
+            out["cpu_baseline"] = cpu_baseline(case, 3 if N > 10_000_000 else 20)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -19,6 +19,8 @@
 
 #include <stdint.h>
 
+#include "lbffs.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -52,6 +54,16 @@ int lbffs_geom_sdf(int n, const float *centroid, const float *normal, float sdf_
  * out, q-major. */
 int lbffs_geom_qlinks(int q, int n, const float *v0, const float *v1, const float *v2, int nx, int ny, int nz,
                       const uint8_t *solid, const uint8_t *neiSolid, float *qlink, int *misses);
+
+/* The same q-distances cast straight into a context's LBFFS_FIELD_QLINK (the lattice,
+ * solid and neiSolid flags are the context's, as uploaded), so that no q*N host array
+ * exists: at 1024 x 384 x 320 cells the D3Q27 array is 13.6 GB.  Links that meet no
+ * triangle, and every link that is not a solid link, are set to init.  The array is
+ * allocated on the first call and counts as uploaded: lbffs_step accepts ibb = 1 after it.
+ * Replaces the setQf buffer fill and upload the reference leaves out (D3Q27_opencl/
+ * ns.cpp:408, walls.hpp:553-584).  Waits for the context's queued work first. */
+int lbffs_geom_qlinks_ctx(lbffs_ctx *ctx, int n, const float *v0, const float *v1, const float *v2, float init,
+                          int *misses);
 
 #ifdef __cplusplus
 }

@@ -119,6 +119,7 @@ def lib():
         L.lbffs_geom_solid.argtypes = [ip, vp, vp, vp, ip, ip, ip, ip, vp]
         L.lbffs_geom_sdf.argtypes = [ip, vp, vp, fl, fl, fl, ip, ip, ip, vp, vp]
         L.lbffs_geom_qlinks.argtypes = [ip, ip, vp, vp, vp, ip, ip, ip, vp, vp, vp, C.POINTER(ip)]
+        L.lbffs_geom_qlinks_ctx.argtypes = [vp, ip, vp, vp, vp, fl, C.POINTER(ip)]
         L.lbffs_case_sdf_near_bwalls.argtypes = [fl, vp, vp, vp, ip, ip, ip, vp]
         L.lbffs_output_macros.argtypes = [vp, fl, vp, vp, vp, vp, sz]
         L.lbffs_out_write_vtk.argtypes = [C.c_char_p, vp, ip]
@@ -133,7 +134,8 @@ def lib():
                   "lbffs_probe_copy", "lbffs_probe_bandwidth", "lbffs_autotune", "lbffs_set_launch",
                   "lbffs_ibm_set_markers", "lbffs_ibm_set_motion", "lbffs_ibm_download", "lbffs_batch_create",
                   "lbffs_batch_context", "lbffs_batch_set_launch", "lbffs_batch_step", "lbffs_batch_last_step_ms",
-                  "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf", "lbffs_geom_qlinks", "lbffs_case_sdf_near_bwalls",
+                  "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf", "lbffs_geom_qlinks", "lbffs_geom_qlinks_ctx",
+                  "lbffs_case_sdf_near_bwalls",
                   "lbffs_output_macros", "lbffs_out_write_vtk", "lbffs_out_write_restart", "lbffs_out_read_restart",
                   "lbffs_out_stats_compute", "lbffs_out_coeffs_fp64", "lbffs_out_accumulate_mean"):
             getattr(L, n).restype = C.c_int

@@ -191,11 +191,11 @@ __device__ inline double geom_link_hit(const GeomTri64 &T, const double *x, cons
 
 // one work-item per listed neiSolid cell; Q directions; q written for the solid links
 // found (no hit keeps the caller's value); misses counted
-template <int Q>
+template <int Q, typename Out>
 __global__ void __launch_bounds__(256)
     geom_qlinks_kernel(const uint32_t *__restrict__ cells, uint32_t ncells, const GeomTri64 *__restrict__ tris,
                        const int *__restrict__ bstart, const int *__restrict__ btri, int bx, int by, int bz, int nx,
-                       int ny, int nz, const uint8_t *__restrict__ solid, float *qlink, uint32_t *misses)
+                       int ny, int nz, const uint8_t *__restrict__ solid, Out *qlink, uint32_t *misses)
 {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= ncells) return;
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(256)
         double best = 2.0;
         for (int t = t0; t < t1; t++) best = fmin(best, geom_link_hit(tris[btri[t]], x, c));
         if (best <= 1.0)
-            qlink[(size_t)d * N + ic] = (float)best;
+            qlink[(size_t)d * N + ic] = (Out)(float)best;  // q rounded to FP32 in either output type
         else
             atomicAdd(misses, 1u);
     }

@@ -1383,6 +1383,114 @@ unsigned geom_grid(size_t n)
     return (unsigned)std::min<size_t>((n + 255) / 256, 65536);
 }
 
+// The surface of a q-distance cast: triangles in FP64 and their buckets. Every
+// kQBucket^3-cell bucket that a triangle's box grown by one cell overlaps lists it, so the
+// links of a cell (at most one cell long per axis) meet only triangles of the cell's bucket.
+struct QlinkSurface {
+    std::vector<lbffs::GeomTri64> tris;
+    std::vector<int> bstart, btri;
+    int bx = 0, by = 0, bz = 0;
+};
+
+int qlink_surface(int n, const float *v0, const float *v1, const float *v2, int nx, int ny, int nz, QlinkSurface &s)
+{
+    const int B = lbffs::kQBucket;
+    s.bx = (nx + B - 1) / B;
+    s.by = (ny + B - 1) / B;
+    s.bz = (nz + B - 1) / B;
+    const size_t nb = (size_t)s.bx * s.by * s.bz;
+    s.tris.resize((size_t)n);
+    s.bstart.assign(nb + 1, 0);
+    std::vector<std::array<int, 6>> range((size_t)n);
+    for (int t = 0; t < n; t++) {
+        double a[3], b[3], c[3];
+        for (int ax = 0; ax < 3; ax++) {
+            a[ax] = v0[(size_t)ax * n + t];
+            b[ax] = v1[(size_t)ax * n + t];
+            c[ax] = v2[(size_t)ax * n + t];
+            s.tris[t].v0[ax] = a[ax];
+            s.tris[t].e1[ax] = b[ax] - a[ax];
+            s.tris[t].e2[ax] = c[ax] - a[ax];
+        }
+        const int dims[3] = {s.bx, s.by, s.bz};
+        bool ok = true;
+        for (int ax = 0; ax < 3; ax++) {
+            const double lo = std::min(a[ax], std::min(b[ax], c[ax])) - 1.0 - 1e-9;
+            const double hi = std::max(a[ax], std::max(b[ax], c[ax])) + 1.0 + 1e-9;
+            if (!(lo <= hi)) ok = false;  // NaN vertex
+            const double l = std::max(std::floor(lo / B), 0.0), h = std::min(std::floor(hi / B), (double)dims[ax] - 1);
+            if (l > h) ok = false;
+            range[t][2 * ax] = ok ? (int)l : 0;
+            range[t][2 * ax + 1] = ok ? (int)h : -1;
+        }
+        if (!ok) range[t] = {0, -1, 0, -1, 0, -1};
+        for (int kk = range[t][4]; kk <= range[t][5]; kk++)
+            for (int jj = range[t][2]; jj <= range[t][3]; jj++)
+                for (int ii = range[t][0]; ii <= range[t][1]; ii++)
+                    s.bstart[(size_t)ii + s.bx * ((size_t)jj + s.by * kk) + 1]++;
+    }
+    for (size_t b = 0; b < nb; b++) s.bstart[b + 1] += s.bstart[b];
+    if ((size_t)s.bstart[nb] > 0x7fffffffu) return fail(LBFFS_E_INVALID, "too many bucket entries");
+    s.btri.resize((size_t)s.bstart[nb]);
+    std::vector<int> fill(s.bstart.begin(), s.bstart.end() - 1);
+    for (int t = 0; t < n; t++)
+        for (int kk = range[t][4]; kk <= range[t][5]; kk++)
+            for (int jj = range[t][2]; jj <= range[t][3]; jj++)
+                for (int ii = range[t][0]; ii <= range[t][1]; ii++)
+                    s.btri[(size_t)fill[(size_t)ii + s.bx * ((size_t)jj + s.by * kk)]++] = t;
+    return 0;
+}
+
+// the fluid cells next to a solid one (neiSolid, walls.hpp:187-216): the cells with solid links
+std::vector<uint32_t> qlink_cells(size_t N, const uint8_t *solid, const uint8_t *neiSolid)
+{
+    std::vector<uint32_t> cells;
+    for (size_t ic = 0; ic < N; ic++)
+        if (neiSolid[ic] == 1 && solid[ic] == 0) cells.push_back((uint32_t)ic);
+    return cells;
+}
+
+// Casts every solid link of the listed cells against the surface; q of the links that meet
+// a triangle is written to d_q (q-major, q*N values), the others keep what d_q holds and
+// are counted in *misses. Returns after the kernel has finished.
+template <typename T>
+int qlink_cast(int q, const QlinkSurface &s, int nx, int ny, int nz, const std::vector<uint32_t> &cells,
+               const uint8_t *d_solid, T *d_q, hipStream_t st, int *misses)
+{
+    GeomScratch g;
+    lbffs::GeomTri64 *d_tris;
+    int *d_bstart, *d_btri;
+    uint32_t *d_cells, *d_miss;
+    if (int e = g.alloc(&d_tris, s.tris.size())) return e;
+    if (int e = g.alloc(&d_bstart, s.bstart.size())) return e;
+    if (int e = g.alloc(&d_btri, s.btri.size())) return e;
+    if (int e = g.alloc(&d_cells, cells.size())) return e;
+    if (int e = g.alloc(&d_miss, 1)) return e;  // zeroed by the allocation
+    if (!s.tris.empty())
+        HIP_TRY(hipMemcpy(d_tris, s.tris.data(), s.tris.size() * sizeof(lbffs::GeomTri64), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_bstart, s.bstart.data(), s.bstart.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!s.btri.empty()) HIP_TRY(hipMemcpy(d_btri, s.btri.data(), s.btri.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!cells.empty())
+        HIP_TRY(hipMemcpy(d_cells, cells.data(), cells.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (!cells.empty()) {
+        const unsigned grid = (unsigned)((cells.size() + 255) / 256);
+        if (q == 27)
+            hipLaunchKernelGGL((lbffs::geom_qlinks_kernel<27, T>), dim3(grid), dim3(256), 0, st, d_cells,
+                               (uint32_t)cells.size(), d_tris, d_bstart, d_btri, s.bx, s.by, s.bz, nx, ny, nz, d_solid,
+                               d_q, d_miss);
+        else
+            hipLaunchKernelGGL((lbffs::geom_qlinks_kernel<19, T>), dim3(grid), dim3(256), 0, st, d_cells,
+                               (uint32_t)cells.size(), d_tris, d_bstart, d_btri, s.bx, s.by, s.bz, nx, ny, nz, d_solid,
+                               d_q, d_miss);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    uint32_t miss = 0;
+    HIP_TRY(hipMemcpy(&miss, d_miss, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (misses) *misses = (int)miss;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1473,89 +1581,47 @@ int lbffs_geom_qlinks(int q, int n, const float *v0, const float *v1, const floa
     if (int e = geom_dims_ok(nx, ny, nz)) return e;
     const size_t N = (size_t)nx * ny * nz;
     if ((double)q * N >= 4294967295.0) return fail(LBFFS_E_INVALID, "lattice too large for the q-distance array");
-    const int B = lbffs::kQBucket;
-    const int bx = (nx + B - 1) / B, by = (ny + B - 1) / B, bz = (nz + B - 1) / B;
-    const size_t nb = (size_t)bx * by * bz;
-    // triangles in FP64 and their buckets: every bucket the box grown by one cell overlaps
-    std::vector<lbffs::GeomTri64> tris((size_t)n);
-    std::vector<int> bstart(nb + 1, 0), btri;
-    std::vector<std::array<int, 6>> range((size_t)n);
-    for (int t = 0; t < n; t++) {
-        double a[3], b[3], c[3];
-        for (int ax = 0; ax < 3; ax++) {
-            a[ax] = v0[(size_t)ax * n + t];
-            b[ax] = v1[(size_t)ax * n + t];
-            c[ax] = v2[(size_t)ax * n + t];
-            tris[t].v0[ax] = a[ax];
-            tris[t].e1[ax] = b[ax] - a[ax];
-            tris[t].e2[ax] = c[ax] - a[ax];
-        }
-        const int dims[3] = {bx, by, bz};
-        bool ok = true;
-        for (int ax = 0; ax < 3; ax++) {
-            const double lo = std::min(a[ax], std::min(b[ax], c[ax])) - 1.0 - 1e-9;
-            const double hi = std::max(a[ax], std::max(b[ax], c[ax])) + 1.0 + 1e-9;
-            if (!(lo <= hi)) ok = false;  // NaN vertex
-            const double l = std::max(std::floor(lo / B), 0.0), h = std::min(std::floor(hi / B), (double)dims[ax] - 1);
-            if (l > h) ok = false;
-            range[t][2 * ax] = ok ? (int)l : 0;
-            range[t][2 * ax + 1] = ok ? (int)h : -1;
-        }
-        if (!ok) range[t] = {0, -1, 0, -1, 0, -1};
-        for (int kk = range[t][4]; kk <= range[t][5]; kk++)
-            for (int jj = range[t][2]; jj <= range[t][3]; jj++)
-                for (int ii = range[t][0]; ii <= range[t][1]; ii++) bstart[(size_t)ii + bx * ((size_t)jj + by * kk) + 1]++;
-    }
-    for (size_t b = 0; b < nb; b++) bstart[b + 1] += bstart[b];
-    if ((size_t)bstart[nb] > 0x7fffffffu) return fail(LBFFS_E_INVALID, "too many bucket entries");
-    btri.resize((size_t)bstart[nb]);
-    {
-        std::vector<int> fill(bstart.begin(), bstart.end() - 1);
-        for (int t = 0; t < n; t++)
-            for (int kk = range[t][4]; kk <= range[t][5]; kk++)
-                for (int jj = range[t][2]; jj <= range[t][3]; jj++)
-                    for (int ii = range[t][0]; ii <= range[t][1]; ii++)
-                        btri[(size_t)fill[(size_t)ii + bx * ((size_t)jj + by * kk)]++] = t;
-    }
-    std::vector<uint32_t> cells;
-    for (size_t ic = 0; ic < N; ic++)
-        if (neiSolid[ic] == 1 && solid[ic] == 0) cells.push_back((uint32_t)ic);
+    QlinkSurface surf;
+    if (int e = qlink_surface(n, v0, v1, v2, nx, ny, nz, surf)) return e;
+    const std::vector<uint32_t> cells = qlink_cells(N, solid, neiSolid);
     GeomScratch g;
-    lbffs::GeomTri64 *d_tris;
-    int *d_bstart, *d_btri;
-    uint32_t *d_cells, *d_miss;
     uint8_t *d_solid;
     float *d_q;
-    if (int e = g.alloc(&d_tris, tris.size())) return e;
-    if (int e = g.alloc(&d_bstart, bstart.size())) return e;
-    if (int e = g.alloc(&d_btri, btri.size())) return e;
-    if (int e = g.alloc(&d_cells, cells.size())) return e;
-    if (int e = g.alloc(&d_miss, 1)) return e;
     if (int e = g.alloc(&d_solid, N)) return e;
     if (int e = g.alloc(&d_q, (size_t)q * N)) return e;
-    if (n) HIP_TRY(hipMemcpy(d_tris, tris.data(), tris.size() * sizeof(lbffs::GeomTri64), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_bstart, bstart.data(), bstart.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (!btri.empty()) HIP_TRY(hipMemcpy(d_btri, btri.data(), btri.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (!cells.empty()) HIP_TRY(hipMemcpy(d_cells, cells.data(), cells.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d_solid, solid, N, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d_q, qlink, (size_t)q * N * sizeof(float), hipMemcpyHostToDevice));
-    if (!cells.empty()) {
-        const unsigned grid = (unsigned)((cells.size() + 255) / 256);
-        if (q == 27)
-            hipLaunchKernelGGL(lbffs::geom_qlinks_kernel<27>, dim3(grid), dim3(256), 0, nullptr, d_cells,
-                               (uint32_t)cells.size(), d_tris, d_bstart, d_btri, bx, by, bz, nx, ny, nz, d_solid, d_q,
-                               d_miss);
-        else
-            hipLaunchKernelGGL(lbffs::geom_qlinks_kernel<19>, dim3(grid), dim3(256), 0, nullptr, d_cells,
-                               (uint32_t)cells.size(), d_tris, d_bstart, d_btri, bx, by, bz, nx, ny, nz, d_solid, d_q,
-                               d_miss);
-        HIP_TRY(hipGetLastError());
-    }
-    uint32_t miss = 0;
-    HIP_TRY(hipMemcpy(&miss, d_miss, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (int e = qlink_cast(q, surf, nx, ny, nz, cells, d_solid, d_q, nullptr, misses)) return e;
     HIP_TRY(hipMemcpy(qlink, d_q, (size_t)q * N * sizeof(float), hipMemcpyDeviceToHost));
-    if (misses) *misses = (int)miss;
     return 0;
+}
+
+int lbffs_geom_qlinks_ctx(lbffs_ctx *c, int n, const float *v0, const float *v1, const float *v2, float init,
+                          int *misses)
+{
+    if (!c || (n > 0 && (!v0 || !v1 || !v2))) return fail(LBFFS_E_INVALID, "null argument");
+    if (n < 0) return fail(LBFFS_E_INVALID, "negative triangle count");
+    const int q = c->d.q, nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    const size_t N = c->N;
+    QlinkSurface surf;
+    if (int e = qlink_surface(n, v0, v1, v2, nx, ny, nz, surf)) return e;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<uint8_t> solid(N), nei(N);
+    HIP_TRY(hipMemcpy(solid.data(), c->solid, N, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(nei.data(), c->neiSolid, N, hipMemcpyDeviceToHost));
+    const std::vector<uint32_t> cells = qlink_cells(N, solid.data(), nei.data());
+    if (!c->qlink) {
+        if (int e = dalloc(&c->qlink, (size_t)q * N * c->real_size)) return e;
+    }
+    if (c->real_size == 8) {
+        hipLaunchKernelGGL((fill_kernel<double>), dim3(4096), dim3(256), 0, c->stream, (double *)c->qlink, (size_t)q * N,
+                           (double)init);
+        HIP_TRY(hipGetLastError());
+        return qlink_cast(q, surf, nx, ny, nz, cells, c->solid, (double *)c->qlink, c->stream, misses);
+    }
+    hipLaunchKernelGGL((fill_kernel<float>), dim3(4096), dim3(256), 0, c->stream, (float *)c->qlink, (size_t)q * N, init);
+    HIP_TRY(hipGetLastError());
+    return qlink_cast(q, surf, nx, ny, nz, cells, c->solid, (float *)c->qlink, c->stream, misses);
 }
 
 int lbffs_geom_sdf(int n, const float *centroid, const float *normal, float sdf_ini, float dr, float p, int nx, int ny,

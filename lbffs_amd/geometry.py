@@ -129,6 +129,76 @@ def read_stl(path: str, L: float) -> STL:
         return parse_stl(fh.read().decode("latin-1"), L)
 
 
+def write_stl(stl: STL, path: str, L: float = 1.0) -> None:
+    """ASCII STL of the patches, vertices and normals times L, in the indented layout the
+    reader's index steps expect (one solid ... endsolid block per patch)."""
+    with open(path, "w") as fh:
+        for p in stl.patches:
+            fh.write(f"solid {p.name}\n")
+            v = [np.asarray(x, np.float64) * L for x in (p.v0, p.v1, p.v2)]
+            nrm = np.asarray(p.normal, np.float64)
+            for t in range(p.n):
+                fh.write("  facet normal %.9g %.9g %.9g\n    outer loop\n" % tuple(nrm[:, t]))
+                for a in v:
+                    fh.write("      vertex %.9g %.9g %.9g\n" % tuple(a[:, t]))
+                fh.write("    endloop\n  endfacet\n")
+            fh.write(f"endsolid {p.name}\n")
+
+
+def procedural_body(centre, R: float, n_lat: int = 250, n_lon: int = 400, amp: float = 0.02, modes: int = 8,
+                    seed: int = 20251202, name: str = "body") -> STL:
+    """A closed, seeded bluff body in lattice units: a sphere of radius R whose radius
+    varies smoothly with the direction n, r(n) = R (1 + amp g(n)) with
+    g = sum_k a_k sin(w_k d_k.n + p_k) / sum_k |a_k| (random unit d_k, w_k in [1, 4]),
+    meshed on n_lat latitude bands (poles on the z axis, longitudes turned by a seeded
+    angle) into 2 n_lon (n_lat - 1) triangles with outward winding; one patch."""
+    rng = np.random.default_rng(seed)
+    d = rng.standard_normal((modes, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    w = rng.uniform(1.0, 4.0, modes)
+    ph = rng.uniform(0.0, 2 * np.pi, modes)
+    a = rng.uniform(-1.0, 1.0, modes)
+    a /= np.abs(a).sum()
+    turn = rng.uniform(0.0, 2 * np.pi / n_lon)
+    th = np.linspace(0.0, np.pi, n_lat + 1)[1:-1]
+    lon = turn + 2 * np.pi * np.arange(n_lon) / n_lon
+    T, P = np.meshgrid(th, lon, indexing="ij")
+    n = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], axis=-1)  # (n_lat-1, n_lon, 3)
+    poles = np.array([[0.0, 0.0, 1.0], [0.0, 0.0, -1.0]])
+
+    def radius(u):
+        return R * (1.0 + amp * np.sin(np.tensordot(u, d.T, axes=1) * w + ph) @ a)
+
+    c = np.asarray(centre, np.float64)
+    ring = c + n * radius(n)[..., None]
+    pole = c + poles * radius(poles)[:, None]
+    b = np.arange(n_lon)
+    b1 = (b + 1) % n_lon
+    tris = [np.stack([np.broadcast_to(pole[0], (n_lon, 3)), ring[0, b], ring[0, b1]], axis=1)]
+    for r in range(n_lat - 2):
+        tris.append(np.stack([ring[r, b], ring[r + 1, b], ring[r + 1, b1]], axis=1))
+        tris.append(np.stack([ring[r, b], ring[r + 1, b1], ring[r, b1]], axis=1))
+    tris.append(np.stack([ring[-1, b], np.broadcast_to(pole[1], (n_lon, 3)), ring[-1, b1]], axis=1))
+    t = np.concatenate(tris).astype(np.float32)       # (n, 3 vertices, 3 axes)
+    v0, v1, v2 = (np.ascontiguousarray(t[:, k, :].T) for k in range(3))
+    cr = np.cross((v1 - v0).T.astype(np.float64), (v2 - v0).T.astype(np.float64))
+    nrm = (cr / np.linalg.norm(cr, axis=1, keepdims=True)).T.astype(np.float32)
+    p = Patch(name, v0, v1, v2, normal=np.ascontiguousarray(nrm))
+    p.c = ((p.v0 + p.v1) + p.v2) / np.float32(3.0)
+    return STL([p])
+
+
+def frontal_area(stl: STL, axis: int = 0) -> float:
+    """Area of a closed surface's projection on the plane normal to `axis`:
+    half the sum of |n_axis| times the triangle areas (FP64)."""
+    tot = 0.0
+    for p in stl.patches:
+        v0, v1, v2 = (np.asarray(x, np.float64) for x in (p.v0, p.v1, p.v2))
+        cr = np.cross((v1 - v0).T, (v2 - v0).T)
+        tot += 0.25 * float(np.abs(cr[:, axis]).sum())
+    return tot
+
+
 def solid_from_triangles(v0, v1, v2, nx: int, ny: int, nz: int, invert: bool = False, solid=None) -> np.ndarray:
     """walls.hpp:159-184 setSolid on the GPU for (3, n) float32 vertex rows; returns the
     nx*ny*nz uint8 flags (a copy of `solid` updated, or fresh zeros)."""
@@ -173,6 +243,17 @@ def qlinks(stl: STL, q: int, nx: int, ny: int, nz: int, solid, nei, init: float 
     _check(lib().lbffs_geom_qlinks(q, v[0].shape[1], *[_ptr(x) for x in v], nx, ny, nz, _ptr(s), _ptr(n8), _ptr(ql),
                                    C.byref(miss)), "lbffs_geom_qlinks")
     return ql, miss.value
+
+
+def qlinks_into(lat, stl: STL, init: float = 0.5) -> int:
+    """The same q-distances cast straight into the lattice's "qlink" field on the device
+    (lbffs_geom_qlinks_ctx; the lattice's uploaded solid and neiSolid flags), without a
+    q*N host array.  Returns the number of solid links that met no triangle."""
+    v = [np.ascontiguousarray(x, np.float32) for x in stl.triangles()]
+    miss = C.c_int()
+    _check(lib().lbffs_geom_qlinks_ctx(lat._h, v[0].shape[1], *[_ptr(x) for x in v], init, C.byref(miss)),
+           "lbffs_geom_qlinks_ctx")
+    return miss.value
 
 
 def sdf_near_bwalls(sdf: np.ndarray, b1, b2, b3, nx: int, ny: int, nz: int, dr: float = BWALL_DR) -> np.ndarray:

@@ -424,3 +424,54 @@ def test_oracle_qlinks_sphere_analytic(q):
         assert err.max() < 0.1 and err.mean() < 0.01, (d, err.max(), err.mean())
         checked += int(hit.sum())
     assert checked > 500
+
+
+def test_procedural_body_mesh():
+    # closed and consistently wound, the triangle count of the latitude-longitude mesh,
+    # the volume and frontal area of the sphere it perturbs, reproducible from its seed
+    c, R = (20.3, 18.4, 16.2), 9.0
+    s = geo.procedural_body(c, R, n_lat=30, n_lon=48, amp=0.02, seed=7)
+    p = s.patches[0]
+    assert p.n == 2 * 48 * 29
+    tri = np.stack([p.v0.T, p.v1.T, p.v2.T], axis=1).astype(np.float64)
+    _, ids = np.unique(tri.reshape(-1, 3), axis=0, return_inverse=True)
+    ids = ids.reshape(-1, 3)
+    edges = np.concatenate([ids[:, [0, 1]], ids[:, [1, 2]], ids[:, [2, 0]]])
+    fwd = {tuple(e) for e in edges}
+    assert len(fwd) == len(edges)                        # every directed edge once
+    assert all((b, a) in fwd for a, b in fwd)            # and its reverse once: closed
+    cc = np.array(c)
+    vol = np.einsum("ij,ij->i", tri[:, 0] - cc, np.cross(tri[:, 1] - cc, tri[:, 2] - cc)).sum() / 6
+    assert abs(vol / (4 / 3 * np.pi * R ** 3) - 1) < 0.03      # outward winding, sphere-like
+    assert abs(geo.frontal_area(s) / (np.pi * R * R) - 1) < 0.03
+    again = geo.procedural_body(c, R, n_lat=30, n_lon=48, amp=0.02, seed=7).patches[0]
+    other = geo.procedural_body(c, R, n_lat=30, n_lon=48, amp=0.02, seed=8).patches[0]
+    assert np.array_equal(again.v0, p.v0) and not np.array_equal(other.v0, p.v0)
+    # the normals are unit and point away from the centre
+    assert np.allclose(np.linalg.norm(p.normal, axis=0), 1.0, atol=1e-6)
+    assert (np.einsum("ij,ij->j", p.normal.astype(np.float64), p.c - cc[:, None]) > 0).all()
+
+
+def test_procedural_body_reference_flags():
+    # the reference's crossing test (oracle) on the procedural mesh gives the body: solid
+    # well inside the perturbed radius, fluid well outside, no streaks of broken parity
+    nx, ny, nz = 40, 36, 32
+    c, R, amp = (20.3, 18.4, 16.2), 9.0, 0.05
+    s = geo.procedural_body(c, R, n_lat=30, n_lon=48, amp=amp, seed=3)
+    o = po.geom_solid(*s.triangles(), nx, ny, nz).reshape(nz, ny, nx)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r = np.sqrt((i - c[0]) ** 2 + (j - c[1]) ** 2 + (k - c[2]) ** 2)
+    assert o[r < R * (1 - amp) - 0.5].all()
+    assert not o[r > R * (1 + amp) + 0.5].any()
+    assert 0.9 < o.sum() / (4 / 3 * np.pi * R ** 3) < 1.1
+
+
+def test_write_stl_round_trip(tmp_path):
+    s = geo.procedural_body((5.5, 6.25, 7.0), 3.0, n_lat=10, n_lon=14, seed=11)
+    path = str(tmp_path / "body.stl")
+    geo.write_stl(s, path)
+    r = geo.read_stl(path, 1.0)
+    assert [p.name for p in r.patches] == ["body"]
+    a, b = s.patches[0], r.patches[0]
+    for k in ("v0", "v1", "v2", "normal", "c"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k

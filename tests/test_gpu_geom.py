@@ -11,6 +11,7 @@ import time
 import numpy as np
 import pytest
 
+import lbffs_amd as lb
 from lbffs_amd import cases
 from lbffs_amd import geometry as geo
 
@@ -194,3 +195,29 @@ def test_qlinks_cylinder_stl():
         side[:3] = False
         side[22:] = False
         assert (g[d][side] >= 0).all(), d
+
+
+def _small_bluff(**kw):
+    return cases.bluff_body(nx=72, ny=40, nz=36, D=18.0, centre=(24.3, 20.4, 18.2), n_lat=40, n_lon=64,
+                            spz_cells=8, Re=100.0, **kw)
+
+
+@pytest.mark.parametrize("storage", [lb.STORE_F32, lb.STORE_BF16DEV, lb.STORE_F64])
+def test_qlinks_into_lattice_equal_host_route(storage):
+    # the q-distances cast straight into a context equal the host-array route bit for bit
+    # (rounded to FP32 in every storage form), and the solid flags of the procedural body
+    # equal the oracle's restatement of setSolid
+    c = _small_bluff(qlinks_on_device=False)
+    stl = c.meta["stl"]
+    o = po.geom_solid(*stl.triangles(), c.nx, c.ny, c.nz)
+    assert np.array_equal(c.arrays["solid"], o)
+    host = c.arrays.pop("qlink")
+    with lb.Lattice(c.nx, c.ny, c.nz, q=27, storage=storage, sgs=c.sgs, collision=lb.COLL_CUMULANT,
+                    omega0=c.omega0) as lat:
+        c.upload_to(lat)
+        miss = geo.qlinks_into(lat, stl)
+        d = lat.download("qlink")
+    print(f"bluff q27 storage {storage}: {int((host != 0.5).sum())} links, misses {miss}", flush=True)
+    assert miss == c.meta["qlink_misses"]
+    assert np.array_equal(d, host.astype(d.dtype))
+    assert (host != 0.5).sum() > 3000

@@ -196,3 +196,15 @@ def test_one_step_rounding_level(q):
     print("one step q", q, "rms ratio hip/o32 by q", np.array2string(rms_h / rms_o, precision=2), flush=True)
     assert np.all(rms_h <= 1.5 * rms_o + 1e-12)
     assert np.abs(h["f"].astype(np.float64) - o32["f"]).max() <= 64 * np.spacing(np.float32(0.3))
+
+
+def test_bluff_body27_qlinks_vs_oracle():
+    # the procedural bluff body: FixedVelocity inlet, Outlet + sponge, Symmetry side faces,
+    # cumulant + WALE, interpolated bounce-back with FP64 q-distances cast from the mesh
+    c = cases.bluff_body(nx=72, ny=40, nz=36, D=18.0, centre=(24.3, 20.4, 18.2), n_lat=40, n_lon=64,
+                         spz_cells=8, Re=100.0, qlinks_on_device=False)
+    h, o, _ = parity(c, 60, label="bluff body q27 qlinks")
+    fw_h = np.array([h[k].astype(np.float64).sum() for k in ("Fwx", "Fwy", "Fwz")])
+    fw_o = np.array([o[k].astype(np.float64).sum() for k in ("Fwx", "Fwy", "Fwz")])
+    assert fw_h[0] > 0
+    assert rel_l2(fw_h, fw_o) <= 1e-4, (fw_h, fw_o)
