@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--forces", type=int, default=0,
                     help="cylinder, bluff27: also run this many steps per storage form (and FP64) and report "
                          "Cd, Cl, St")
+    ap.add_argument("--forces-ref", default="f64", choices=["f64", "f32"],
+                    help="storage form the other forms' coefficients are compared with")
     return ap.parse_args()
 
 
@@ -210,11 +212,16 @@ def force_history(case, storage, nsteps, every=50):
     D, u = case.meta["D"], case.meta["u_in"]
     norm = 0.5 * u * u * case.meta.get("frontal_area", D * case.nz)
     cd, cl = [], []
-    for _ in range(nsteps // every):
+    t0 = time.perf_counter()
+    t_log = t0
+    for n in range(nsteps // every):
         lat.step(every, **case.params)
         s = lat.reduce()
         cd.append(s[0] / norm)
         cl.append(s[1] / norm)
+        if time.perf_counter() - t_log > 30:
+            t_log = time.perf_counter()
+            log(f"[bench]   step {(n + 1) * every}: Cd {cd[-1]:.4f} Cl {cl[-1]:.4f} ({t_log - t0:.0f} s)")
     lat.close()
     return np.array(cd), np.array(cl)
 
@@ -233,22 +240,27 @@ def shedding(cd, cl, every, D, u):
     return {"Cd": float(cd[h:].mean()), "Cl_rms": float(np.sqrt(np.mean(c * c))), "St": float(f * D / u)}
 
 
-def forces_report(case, nsteps, forms):
+def forces_report(case, nsteps, forms, ref_name="f64"):
     every = 50
     D, u = case.meta["D"], case.meta["u_in"]
     out = {}
-    for name, st in list(forms.items()) + [("f64", lb.STORE_F64)]:
+    runs = dict(forms)
+    runs.setdefault(ref_name, lb.STORE_F64 if ref_name == "f64" else lb.STORE_F32)
+    for name, st in runs.items():
         log(f"[bench] forces {name}: {nsteps} steps")
         cd, cl = force_history(case, st, nsteps, every)
         out[name] = shedding(cd, cl, every, D, u)
-    ref = out["f64"]
-    for name in forms:
+    ref = out[ref_name]
+    for name in runs:
+        if name == ref_name:
+            continue
         r = out[name]
-        r["Cd_rel_err_vs_f64"] = abs(r["Cd"] - ref["Cd"]) / abs(ref["Cd"])
-        r["St_rel_err_vs_f64"] = abs(r["St"] - ref["St"]) / abs(ref["St"]) if ref["St"] else None
-        r["Cl_rms_rel_err_vs_f64"] = abs(r["Cl_rms"] - ref["Cl_rms"]) / max(abs(ref["Cl_rms"]), 1e-12)
+        r[f"Cd_rel_err_vs_{ref_name}"] = abs(r["Cd"] - ref["Cd"]) / abs(ref["Cd"])
+        r[f"St_rel_err_vs_{ref_name}"] = abs(r["St"] - ref["St"]) / abs(ref["St"]) if ref["St"] else None
+        r[f"Cl_rms_rel_err_vs_{ref_name}"] = abs(r["Cl_rms"] - ref["Cl_rms"]) / max(abs(ref["Cl_rms"]), 1e-12)
     out["steps"] = nsteps
     out["sample_every"] = every
+    out["reference_form"] = ref_name
     return out
 
 
@@ -349,7 +361,7 @@ def main():
         out["config"]["qlink_misses"] = case.meta.get("qlink_misses")
         out["config"]["triangles"] = case.meta.get("triangles")
     if a.forces and a.workload in ("cylinder", "bluff27"):
-        out["forces"] = forces_report(case, a.forces, {k: forms[k] for k in run})
+        out["forces"] = forces_report(case, a.forces, {k: forms[k] for k in run}, a.forces_ref)
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (oracle on host cores)")
         if a.workload == "bluff27":

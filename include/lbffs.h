@@ -138,7 +138,10 @@ typedef struct {
 typedef struct lbffs_ctx lbffs_ctx;
 
 /* Create a context on the current HIP device (the OpenCL context + buffers of
- * ns.cpp:385-436).  Every field is zero-initialised except f, which must be uploaded. */
+ * ns.cpp:385-436).  Every field is zero-initialised except f, which must be uploaded.
+ * One population array (N cells in the storage form) and one macroscopic array (N FP32
+ * values, FP64 for LBFFS_STORE_F64) must each stay below 4 GiB: about 1.07e9 cells for
+ * FP32 and the 16-bit forms, 5.4e8 for FP64 (LBFFS_E_INVALID otherwise). */
 int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out);
 
 /* Copy a host array into a device field (the cl::Buffer fills of ns.cpp:405-436).

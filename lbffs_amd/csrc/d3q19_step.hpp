@@ -256,20 +256,11 @@ __device__ inline Real grad_comp(const Real *old, Real self, long ic, int i, int
 #undef AT
 }
 
-// kernel.cl:2267-2365 waleTauSGS (Cw = 0.325)
-template <typename Real, int Q>
-__device__ inline Real wale_tau(const StepArgs<Real, Q> &a, long ic, int i, int j, int k, Real u, Real v, Real w)
+// kernel.cl:2267-2365 waleTauSGS (Cw = 0.325) from the velocity gradient
+template <typename Real>
+__device__ inline Real wale_from_grad(Real dudx, Real dudy, Real dudz, Real dvdx, Real dvdy, Real dvdz, Real dwdx,
+                                      Real dwdy, Real dwdz)
 {
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    Real dudx = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 0);
-    Real dudy = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 1);
-    Real dudz = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 2);
-    Real dvdx = grad_comp<Real>(a.v_o, v, ic, i, j, k, nx, ny, nz, 0);
-    Real dvdy = grad_comp<Real>(a.v_o, v, ic, i, j, k, nx, ny, nz, 1);
-    Real dvdz = grad_comp<Real>(a.v_o, v, ic, i, j, k, nx, ny, nz, 2);
-    Real dwdx = grad_comp<Real>(a.w_o, w, ic, i, j, k, nx, ny, nz, 0);
-    Real dwdy = grad_comp<Real>(a.w_o, w, ic, i, j, k, nx, ny, nz, 1);
-    Real dwdz = grad_comp<Real>(a.w_o, w, ic, i, j, k, nx, ny, nz, 2);
     const Real Cw = Real(0.325);
     Real S12 = Real(0.5) * (dudy + dvdx), S13 = Real(0.5) * (dudz + dwdx), S23 = Real(0.5) * (dvdz + dwdy);
     Real SS = dudx * dudx + Real(2) * S12 * S12 + Real(2) * S13 * S13 + dvdy * dvdy + Real(2) * S23 * S23
@@ -293,6 +284,39 @@ __device__ inline Real wale_tau(const StepArgs<Real, Q> &a, long ic, int i, int 
     Real ss25 = SS * SS * sqrt(SS);         // SS^2.5
     Real sd125 = SdSd * sqrt(sqrt(SdSd));        // SdSd^1.25
     return Real(9) * (Cw * Cw * sd15 / (ss25 + sd125));
+}
+
+// central difference 0.5 (p[o + d] - p[o - d]) at byte offset o (32-bit offsets from the
+// array base; d = one element for x, so that it folds into the load's immediate offset)
+template <typename Real>
+__device__ inline Real cdiff(const Real *p, uint32_t o, uint32_t d)
+{
+    const char *b = (const char *)p;
+    return Real(0.5) * (*(const Real *)(b + (size_t)(o + d)) - *(const Real *)(b + (size_t)(o - d)));
+}
+
+template <typename Real, int Q>
+__device__ inline Real wale_tau(const StepArgs<Real, Q> &a, long ic, int i, int j, int k, Real u, Real v, Real w)
+{
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && k > 0 && k < nz - 1) {
+        // all six neighbours exist: central differences only, as grad_comp takes them there
+        const uint32_t o = (uint32_t)ic * (uint32_t)sizeof(Real);
+        const uint32_t dx = (uint32_t)sizeof(Real), dy = (uint32_t)nx * dx, dz = (uint32_t)ny * dy;
+        const Real *U = a.u_o, *V = a.v_o, *W = a.w_o;
+        return wale_from_grad<Real>(cdiff(U, o, dx), cdiff(U, o, dy), cdiff(U, o, dz), cdiff(V, o, dx), cdiff(V, o, dy),
+                                    cdiff(V, o, dz), cdiff(W, o, dx), cdiff(W, o, dy), cdiff(W, o, dz));
+    }
+    Real dudx = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 0);
+    Real dudy = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 1);
+    Real dudz = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 2);
+    Real dvdx = grad_comp<Real>(a.v_o, v, ic, i, j, k, nx, ny, nz, 0);
+    Real dvdy = grad_comp<Real>(a.v_o, v, ic, i, j, k, nx, ny, nz, 1);
+    Real dvdz = grad_comp<Real>(a.v_o, v, ic, i, j, k, nx, ny, nz, 2);
+    Real dwdx = grad_comp<Real>(a.w_o, w, ic, i, j, k, nx, ny, nz, 0);
+    Real dwdy = grad_comp<Real>(a.w_o, w, ic, i, j, k, nx, ny, nz, 1);
+    Real dwdz = grad_comp<Real>(a.w_o, w, ic, i, j, k, nx, ny, nz, 2);
+    return wale_from_grad<Real>(dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz);
 }
 
 // reciprocal: one v_rcp_f32 in FP32 (<= 1 ulp), an exact quotient in FP64

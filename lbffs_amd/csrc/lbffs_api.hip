@@ -887,12 +887,15 @@ int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out)
         return fail(LBFFS_E_UNSUPPORTED, "D3Q27 runs the cumulant collision (K27:2824)");
     if (desc->block != 0 && (desc->block % 64 != 0 || desc->block > 256 || desc->block < 64))
         return fail(LBFFS_E_INVALID, "block must be 64, 128, 192 or 256");
-    // the step kernels address a population array with 32-bit byte offsets from its base
+    // the step kernels address a population array, and the interior velocity gradient a
+    // macroscopic array (FP32, FP64 for FP64 storage), with 32-bit byte offsets from its base
     {
         const long elem = (desc->storage == LBFFS_STORE_F64) ? 8 : (desc->storage == LBFFS_STORE_F32 ? 4 : 2);
-        if (N * elem > 0xFFFFFFFFL)
-            return fail(LBFFS_E_INVALID, "one population array must stay below 4 GiB (" + std::to_string(N) +
-                                             " cells of " + std::to_string(elem) + " bytes)");
+        const long real = (desc->storage == LBFFS_STORE_F64) ? 8 : 4;
+        if (N * std::max(elem, real) > 0xFFFFFFFFL)
+            return fail(LBFFS_E_INVALID, "one population or macroscopic array must stay below 4 GiB (" +
+                                             std::to_string(N) + " cells of " + std::to_string(std::max(elem, real)) +
+                                             " bytes)");
     }
 
     lbffs_ctx *c = new lbffs_ctx();

@@ -64,6 +64,10 @@ def test_strerror_and_invalid_arguments():
     d = lb.Desc(abi_version=lb.ABI_VERSION, q=19, nx=1100, ny=1100, nz=1100, storage=lb.STORE_F32)
     assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert b"4 GiB" in L.lbffs_last_error()
+    # and so is a macroscopic (FP32) array: the same lattice in FP16 storage
+    d = lb.Desc(abi_version=lb.ABI_VERSION, q=19, nx=1100, ny=1100, nz=1100, storage=lb.STORE_F16DEV)
+    assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"4 GiB" in L.lbffs_last_error()
     # geometry: argument checks come before any device call
     assert L.lbffs_geom_solid(1, None, None, None, 4, 4, 4, 0, None) == -1
     assert L.lbffs_geom_sdf(1, None, None, 1.0, 1.0, 7.0, 4, 4, 4, None, None) == -1
