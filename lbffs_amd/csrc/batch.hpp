@@ -22,8 +22,10 @@ constexpr int kBatchBlock = 256;
 // The launch arguments of every lattice are read from device memory where they are used
 // (uniform per work-group, so through scalar loads); the host prepares a separate argument
 // array for the last step of a call, whose cells store every moment and diagnostic.
-template <typename S, typename Real, int SGS, int Q>
-__global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Q == 27 ? kStep27MinWaves<Real> : 1, 8)))
+// IBB: D3Q19's interpolated bounce-back compiled in (chosen when any lattice of the batch
+// asks for it); the lattices without it keep simple bounce-back through their own ibb flag
+template <typename S, typename Real, int SGS, int Q, bool IBB>
+__global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Q == 27 ? kBatch27MinWaves<Real> : 1, 8)))
     batch_step_kernel(const StepArgs<Real, Q> *__restrict__ args, const BatchBlock *__restrict__ table,
                       uint32_t nblocks)
 {
@@ -34,12 +36,14 @@ __global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_e
         if (icu >= a.N) continue;
         do {
 #define CELL_RETURN continue
-#define CELL_IBB (a.ibb != 0)
+#define CELL_IBB (IBB && a.ibb != 0)
+#define CELL_WALE_INTERIOR false
             if constexpr (Q == 27) {
 #include "d3q27_cell.inc"
             } else {
 #include "d3q19_cell.inc"
             }
+#undef CELL_WALE_INTERIOR
 #undef CELL_IBB
 #undef CELL_RETURN
         } while (0);

@@ -59,7 +59,8 @@ struct StepArgs {
     int ibb;            // interpolated bounce-back with qlink instead of the reference's walls
     int sround;         // 16-bit forms: stochastic rounding of the stored deviations
     int damping;
-    int mac_all;      // write rho/u/v/w for every fluid cell
+    int mac_all;      // write u/v/w for every fluid cell
+    int rho_all;      // write rho for every fluid cell (implies mac_all)
     int aux_all;      // write tauSGS / Fw / G for every fluid cell
     int sponge_mask;  // faces whose centre cell is Outlet or FixedDensity (tauSpongeZone)
     Real inv_spx, inv_spy, inv_spz;  // 1 / (n * spzWidth) per axis
@@ -295,11 +296,13 @@ __device__ inline Real cdiff(const Real *p, uint32_t o, uint32_t d)
     return Real(0.5) * (*(const Real *)(b + (size_t)(o + d)) - *(const Real *)(b + (size_t)(o - d)));
 }
 
-template <typename Real, int Q>
+// Interior: take the central differences of interior cells directly (the single-lattice
+// kernels; in the batch kernel the extra path costs registers, one wave per SIMD)
+template <bool Interior, typename Real, int Q>
 __device__ inline Real wale_tau(const StepArgs<Real, Q> &a, long ic, int i, int j, int k, Real u, Real v, Real w)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && k > 0 && k < nz - 1) {
+    if (Interior && i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && k > 0 && k < nz - 1) {
         // all six neighbours exist: central differences only, as grad_comp takes them there
         const uint32_t o = (uint32_t)ic * (uint32_t)sizeof(Real);
         const uint32_t dx = (uint32_t)sizeof(Real), dy = (uint32_t)nx * dx, dz = (uint32_t)ny * dy;
@@ -661,7 +664,9 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
     if (icu >= a.N) return;
 #define CELL_RETURN return
 #define CELL_IBB IBB
+#define CELL_WALE_INTERIOR true
 #include "d3q19_cell.inc"
+#undef CELL_WALE_INTERIOR
 #undef CELL_IBB
 #undef CELL_RETURN
 }

@@ -522,10 +522,14 @@ __device__ inline void cumulant27(Real *ft, Real *K, Real rho, Real u, Real v, R
     for (int q = 0; q < 27; q++) ft[q] = K[L27::slot[q]];
 }
 
-// At least six waves per SIMD for the 32-bit arithmetic: the cumulant update is VALU bound
-// and fits 64 VGPRs without spilling; the FP64 form keeps the compiler's own choice.
+// At least seven waves per SIMD (72 VGPRs) for the 32-bit arithmetic: the cumulant update
+// is VALU bound and fits without spilling (eight waves, 64 VGPRs, would spill); the
+// compiler's own choice went as low as five.  The FP64 form keeps the compiler's choice.
 template <typename Real>
-constexpr int kStep27MinWaves = sizeof(Real) == 8 ? 1 : 6;
+constexpr int kStep27MinWaves = sizeof(Real) == 8 ? 1 : 7;
+// the batch kernel's extra indirection spills at seven: six
+template <typename Real>
+constexpr int kBatch27MinWaves = sizeof(Real) == 8 ? 1 : 6;
 
 template <typename S, typename Real, int SGS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStep27MinWaves<Real>, 8)))
@@ -535,7 +539,9 @@ step27_kernel(const StepArgs<Real, 27> a)
     const uint32_t icu = blk * blockDim.x + threadIdx.x;
     if (icu >= a.N) return;
 #define CELL_RETURN return
+#define CELL_WALE_INTERIOR true
 #include "d3q27_cell.inc"
+#undef CELL_WALE_INTERIOR
 #undef CELL_RETURN
 }
 

@@ -552,9 +552,10 @@ int run_steps_q(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
         a.v_n = (Real *)c->v[1 - c->mcur];
         a.w_n = (Real *)c->w[1 - c->mcur];
         a.aux_all = (p->store_every || last) ? 1 : 0;
-        // WALE reads every neighbour's velocity; the moving-wall kernels read the
-        // velocities of the marker boxes and the density of the cells they force
+        // WALE reads every neighbour's velocity (not its density); the moving-wall kernels
+        // read the velocities of the marker boxes and the density of the cells they force
         a.mac_all = (a.aux_all || c->d.sgs == LBFFS_SGS_WALE || ibm) ? 1 : 0;
+        a.rho_all = (a.aux_all || ibm) ? 1 : 0;
         switch (c->d.sgs) {
         case LBFFS_SGS_WALE: launch_step<S, Real, Q, 0>(a, grid, block, c->stream); break;
         case LBFFS_SGS_SMAGORINSKY: launch_step<S, Real, Q, 1>(a, grid, block, c->stream); break;
@@ -738,6 +739,7 @@ struct lbffs_batch {
     int graph_chunk = 0; // steps per captured hipGraph; 0 = plain launches
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int gexec_chunk[2] = {0, 0};
+    int gexec_ibb[2] = {0, 0};  // kernel variant the graph was captured with
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
 };
@@ -753,16 +755,16 @@ void batch_drop_graphs(lbffs_batch *b)
     }
 }
 
-template <typename S, typename Real, int Q, int SGS>
+template <typename S, typename Real, int Q, int SGS, bool IBB>
 void batch_launch(lbffs_batch *b, int parity, int last)
 {
     const unsigned grid = b->grid > 0 ? (unsigned)b->grid : b->nblocks;
-    hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q>), dim3(grid), dim3(kBatchBlock), 0, b->stream,
+    hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, IBB>), dim3(grid), dim3(kBatchBlock), 0, b->stream,
                        (const StepArgs<Real, Q> *)b->args_dev[parity + 2 * last], (const BatchBlock *)b->table,
                        b->nblocks);
 }
 
-template <typename S, typename Real, int Q, int SGS>
+template <typename S, typename Real, int Q, int SGS, bool IBB>
 int batch_steps_sgs(lbffs_batch *b, int nsteps, int cur0)
 {
     int s = 0;
@@ -771,17 +773,18 @@ int batch_steps_sgs(lbffs_batch *b, int nsteps, int cur0)
         const int chunk = b->graph_chunk;
         while (nsteps - 1 - s >= chunk) {
             const int p = (cur0 + s) & 1;
-            if (!b->gexec[p] || b->gexec_chunk[p] != chunk) {
+            if (!b->gexec[p] || b->gexec_chunk[p] != chunk || b->gexec_ibb[p] != (int)IBB) {
                 if (b->gexec[p]) (void)hipGraphExecDestroy(b->gexec[p]);
                 b->gexec[p] = nullptr;
                 hipGraph_t g = nullptr;
                 HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-                for (int k = 0; k < chunk; k++) batch_launch<S, Real, Q, SGS>(b, (p + k) & 1, 0);
+                for (int k = 0; k < chunk; k++) batch_launch<S, Real, Q, SGS, IBB>(b, (p + k) & 1, 0);
                 HIP_TRY(hipStreamEndCapture(b->stream, &g));
                 hipError_t e = hipGraphInstantiate(&b->gexec[p], g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 if (e != hipSuccess) return fail(LBFFS_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
                 b->gexec_chunk[p] = chunk;
+                b->gexec_ibb[p] = (int)IBB;
             }
             HIP_TRY(hipGraphLaunch(b->gexec[p], b->stream));
             s += chunk;
@@ -789,7 +792,7 @@ int batch_steps_sgs(lbffs_batch *b, int nsteps, int cur0)
     }
     for (; s < nsteps; s++) {
         const int p = (cur0 + s) & 1;
-        batch_launch<S, Real, Q, SGS>(b, p, s == nsteps - 1 ? 1 : 0);
+        batch_launch<S, Real, Q, SGS, IBB>(b, p, s == nsteps - 1 ? 1 : 0);
         HIP_TRY(hipGetLastError());
     }
     return 0;
@@ -819,13 +822,27 @@ int batch_steps_q(lbffs_batch *b, int nsteps, const lbffs_step_params *params)
             a.w_n = (Real *)c->w[1 - p];
             a.aux_all = (params[d].store_every || last) ? 1 : 0;
             a.mac_all = (a.aux_all || c->d.sgs == LBFFS_SGS_WALE) ? 1 : 0;
+            a.rho_all = a.aux_all;
         }
         HIP_TRY(hipMemcpy(b->args_dev[v], b->args_host[v].data(), b->args_host[v].size(), hipMemcpyHostToDevice));
     }
+    // D3Q19's interpolated bounce-back is a kernel variant (its loop costs registers in
+    // every cell); D3Q27 selects the q source per link at run time
+    bool ibb = false;
+    for (size_t d = 0; d < b->ctx.size(); d++) ibb = ibb || params[d].ibb;
+    if constexpr (Q == 19) {
+        if (ibb) {
+            switch (b->ctx[0]->d.sgs) {
+            case LBFFS_SGS_WALE: return batch_steps_sgs<S, Real, Q, 0, true>(b, nsteps, cur0);
+            case LBFFS_SGS_SMAGORINSKY: return batch_steps_sgs<S, Real, Q, 1, true>(b, nsteps, cur0);
+            default: return batch_steps_sgs<S, Real, Q, 2, true>(b, nsteps, cur0);
+            }
+        }
+    }
     switch (b->ctx[0]->d.sgs) {
-    case LBFFS_SGS_WALE: return batch_steps_sgs<S, Real, Q, 0>(b, nsteps, cur0);
-    case LBFFS_SGS_SMAGORINSKY: return batch_steps_sgs<S, Real, Q, 1>(b, nsteps, cur0);
-    default: return batch_steps_sgs<S, Real, Q, 2>(b, nsteps, cur0);
+    case LBFFS_SGS_WALE: return batch_steps_sgs<S, Real, Q, 0, false>(b, nsteps, cur0);
+    case LBFFS_SGS_SMAGORINSKY: return batch_steps_sgs<S, Real, Q, 1, false>(b, nsteps, cur0);
+    default: return batch_steps_sgs<S, Real, Q, 2, false>(b, nsteps, cur0);
     }
 }
 
