@@ -2,7 +2,8 @@
 """Throughput of the fused D3Q19 time step on one MI355X, in the reference's metric.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload backstep|cylinder|cavity|box27|bluff27|ensemble]
+                    [--workload backstep|cylinder|cavity|box19|box27|bluff27|ensemble]
+                    [--streaming two-lattice|aa]
 
 Default workload: backward-facing step, D3Q19, Smagorinsky LES (Cs=0.1, wall damping),
 1536 x 192 x 128 lattice (37.7M cells), FixedVelocity inlet with seeded 5 % Gaussian
@@ -40,7 +41,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="backstep", choices=["backstep", "cylinder", "cavity", "box27", "bluff27", "ensemble"])
+    ap.add_argument("--workload", default="backstep", choices=["backstep", "cylinder", "cavity", "box19", "box27", "bluff27", "ensemble"])
     ap.add_argument("--domains", type=int, default=512, help="ensemble: number of lattices")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
@@ -48,6 +49,10 @@ def parse():
     ap.add_argument("--forces", type=int, default=0,
                     help="cylinder, bluff27: also run this many steps per storage form (and FP64) and report "
                          "Cd, Cl, St")
+    ap.add_argument("--streaming", default="two-lattice", choices=["two-lattice", "aa"],
+                    help="population streaming: two arrays, or in place (AA; D3Q19, Cyclic/BounceBack faces)")
+    ap.add_argument("--rounding", default="stochastic", choices=["stochastic", "nearest"],
+                    help="rounding of the 16-bit storage forms")
     ap.add_argument("--forces-ref", default="f64", choices=["f64", "f32"],
                     help="storage form the other forms' coefficients are compared with")
     return ap.parse_args()
@@ -60,6 +65,10 @@ def build_case(name):
         # channel past a cylinder: 400 x 160 x 16, D = 40 at (100, 80), inlet 0.05 with 1e-3
         # noise, outlet + 20-cell sponge, Re = 100, interpolated bounce-back with analytic q
         return cases.cylinder_channel(ibb=True)
+    if name == "box19":
+        # D3Q19 on a periodic 384 x 384 x 256 box (37.7M cells, the backstep's count), no
+        # sub-grid model: the lattice in-place streaming can run
+        return cases.periodic_box(384, 384, 256, tau=0.8, ini=(0.02, 0.01, 0.0), sgs=lb.SGS_NONE)
     if name == "box27":
         # D3Q27 cumulant step on a periodic 250^3 box (15.6M cells), no sub-grid model
         return cases.periodic_box(250, 250, 250, tau=0.8, sgs=lb.SGS_NONE, q=27)
@@ -74,11 +83,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(case, storage, warmup, steps, tune=True):
+ROUNDING = {"stochastic": lb.ROUND_STOCHASTIC, "nearest": lb.ROUND_NEAREST}
+
+
+STREAMING = {"two-lattice": lb.STREAM_TWO_LATTICE, "aa": lb.STREAM_AA}
+
+
+def run_form(case, storage, warmup, steps, tune=True, rounding=lb.ROUND_STOCHASTIC, streaming=lb.STREAM_TWO_LATTICE):
     """Warm up, then time exactly `steps` steps bracketed by device synchronisation."""
     coll = lb.COLL_CUMULANT if case.q == 27 else lb.COLL_RR
     lat = lb.Lattice(case.nx, case.ny, case.nz, q=case.q, storage=storage, sgs=case.sgs, collision=coll,
-                     omega0=case.omega0)
+                     omega0=case.omega0, rounding=rounding, streaming=streaming)
     case.upload_to(lat)
     lat.tuned = lat.autotune(10, **case.params) if tune else None
     if warmup:
@@ -285,7 +300,8 @@ def main():
     vel = {}
     for name in run:
         log(f"[bench] {name}: warmup {a.warmup}, timed {a.steps}")
-        lat, wall, dev_ms = run_form(case, forms[name], a.warmup, a.steps)
+        lat, wall, dev_ms = run_form(case, forms[name], a.warmup, a.steps, rounding=ROUNDING[a.rounding],
+                                     streaming=STREAMING[a.streaming])
         bytes_per_cell = 2 * case.q * (4 if name == "f32" else 2)
         launch_ms = dev_ms / a.steps
         res[name] = {"wall_s": wall, "dev_ms": dev_ms, "launch_ms": launch_ms, "launch": lat.tuned,
@@ -333,13 +349,15 @@ def main():
         "config": {"workload": {"backstep": "backward-facing step D3Q19 Smagorinsky LES Re_H=5500",
                                 "cylinder": "channel past cylinder D3Q19 Re=100, interpolated bounce-back with analytic q",
                                 "cavity": "lid-driven cavity D3Q19 Re=100",
+                                "box19": "periodic box D3Q19 recursive-regularized, no SGS",
                                 "box27": "periodic box D3Q27 cumulant, no SGS",
                                 "bluff27": "bluff body D3Q27 cumulant WALE Re=300, procedural STL (199,200 "
                                            "triangles), FP64 q-distances, interpolated bounce-back"}[a.workload],
                    "lattice": [case.nx, case.ny, case.nz], "cells": N, "fluid_cells": fluid,
                    "collision": "cumulant" if case.q == 27 else "recursive-regularized",
                    "sgs": {0: "WALE", 1: "Smagorinsky", 2: "none"}[case.sgs],
-                   "storage": main_form, "parallelism": "single GPU"},
+                   "storage": main_form, "rounding_16bit": a.rounding, "streaming": a.streaming,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_Bps"] / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(r["achieved_Bps"] / HBM_PEAK, 4),
                      "traffic": traffic,

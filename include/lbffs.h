@@ -105,7 +105,8 @@ typedef struct {
     float omega0;  /* 1/(3 nu + 1/2): cells whose omega equals it skip the omega read */
     int block;     /* work-group size; 0 = engine default */
     int rounding;  /* 16-bit storage forms: LBFFS_ROUND_STOCHASTIC (default) or LBFFS_ROUND_NEAREST */
-    int reserved[7];
+    int streaming; /* LBFFS_STREAM_TWO_LATTICE (default) or LBFFS_STREAM_AA */
+    int reserved[6];
 } lbffs_desc;
 
 /* Rounding of the 16-bit storage forms (no counterpart in the reference, which stores
@@ -114,6 +115,16 @@ typedef struct {
  * at Re 100 never sheds); stochastic rounding (gfx950 v_cvt_sr_*) is unbiased, with the
  * random bits hashed from the cell's moments: deterministic for a given state. */
 enum { LBFFS_ROUND_STOCHASTIC = 0, LBFFS_ROUND_NEAREST = 1 };
+
+/* Streaming.  TWO_LATTICE pulls from one population array into another and swaps them,
+ * as ns.cpp:602 does.  AA streams in place on one array (half the population memory):
+ * odd and even steps alternate two kernels that each read and write the same locations
+ * per cell; results are bit-identical to TWO_LATTICE.  AA runs D3Q19 lattices whose faces
+ * are Cyclic or BounceBack with the same code on both sides of an axis, without the
+ * interpolated bounce-back (ibb) and moving walls (LBFFS_E_UNSUPPORTED / LBFFS_E_INVALID
+ * otherwise).  Downloads, lbffs_output_macros and lbffs_reduce see the two-lattice layout
+ * whatever the step count (after an odd step they first restore it in place). */
+enum { LBFFS_STREAM_TWO_LATTICE = 0, LBFFS_STREAM_AA = 1 };
 
 /* Scalars of one k_streamingCollision launch (D3Q19 ns.cpp:528-535, K19:3541-3545;
  * D3Q27 ns.cpp:555-577, K27:2725-2731). */

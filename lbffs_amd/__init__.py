@@ -23,6 +23,7 @@ STORE_F32, STORE_F16DEV, STORE_BF16DEV, STORE_F64 = 0, 1, 2, 3
 SGS_WALE, SGS_SMAGORINSKY, SGS_NONE = 0, 1, 2
 COLL_RR, COLL_CUMULANT = 0, 1
 ROUND_STOCHASTIC, ROUND_NEAREST = 0, 1  # 16-bit storage forms
+STREAM_TWO_LATTICE, STREAM_AA = 0, 1   # population streaming: two arrays, or in place
 
 # BC codes (input.hpp:221-256)
 BC = {"Cyclic": 0, "BounceBack": 1, "Outlet": 3, "Symmetry": 4, "FixedVelocity": 5,
@@ -44,7 +45,7 @@ class LbffsError(RuntimeError):
 class Desc(C.Structure):
     _fields_ = [("abi_version", C.c_int), ("q", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("storage", C.c_int), ("sgs", C.c_int), ("collision", C.c_int), ("omega0", C.c_float),
-                ("block", C.c_int), ("rounding", C.c_int), ("reserved", C.c_int * 7)]
+                ("block", C.c_int), ("rounding", C.c_int), ("streaming", C.c_int), ("reserved", C.c_int * 6)]
 
 
 class StepParams(C.Structure):
@@ -176,13 +177,13 @@ class Lattice:
     """One device context (lbffs_create .. lbffs_destroy)."""
 
     def __init__(self, nx, ny, nz, q=19, storage=STORE_F32, sgs=SGS_WALE, collision=COLL_RR, omega0=1.0,
-                 block=0, rounding=0):
+                 block=0, rounding=0, streaming=STREAM_TWO_LATTICE):
         self.nx, self.ny, self.nz, self.q = nx, ny, nz, q
         self.N = nx * ny * nz
         self.storage = storage
         self.real = np.float64 if storage == STORE_F64 else np.float32
         d = Desc(abi_version=ABI_VERSION, q=q, nx=nx, ny=ny, nz=nz, storage=storage, sgs=sgs,
-                 collision=collision, omega0=omega0, block=block, rounding=rounding)
+                 collision=collision, omega0=omega0, block=block, rounding=rounding, streaming=streaming)
         h = C.c_void_p()
         _check(lib().lbffs_create(C.byref(d), C.byref(h)), "lbffs_create")
         self._h = h

@@ -38,11 +38,13 @@ __global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_e
 #define CELL_RETURN continue
 #define CELL_IBB (IBB && a.ibb != 0)
 #define CELL_WALE_INTERIOR false
+#define CELL_AA 0
             if constexpr (Q == 27) {
 #include "d3q27_cell.inc"
             } else {
 #include "d3q19_cell.inc"
             }
+#undef CELL_AA
 #undef CELL_WALE_INTERIOR
 #undef CELL_IBB
 #undef CELL_RETURN

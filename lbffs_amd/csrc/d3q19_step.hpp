@@ -655,8 +655,9 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
 
 // IBB selects the interpolated bounce-back at compile time: its loop raises the register
 // count of the whole kernel from 63 to 72 VGPRs (7 instead of 8 waves per SIMD), which the
-// 16-bit forms, bound by instruction issue, feel.
-template <typename S, typename Real, int SGS, bool IBB = false>
+// 16-bit forms, bound by instruction issue, feel.  AA: 0 two-lattice streaming, 1 and 2
+// the two half-steps of in-place streaming (the cell body describes them).
+template <typename S, typename Real, int SGS, bool IBB = false, int AA = 0>
 __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
 {
     const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
@@ -665,10 +666,48 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
 #define CELL_RETURN return
 #define CELL_IBB IBB
 #define CELL_WALE_INTERIOR true
+#define CELL_AA AA
 #include "d3q19_cell.inc"
+#undef CELL_AA
 #undef CELL_WALE_INTERIOR
 #undef CELL_IBB
 #undef CELL_RETURN
+}
+
+// In-place streaming: after the first half-step, out[p] of cell x sits in slot opp(p) of
+// x + c_p and out[opp(p)] of x + c_p in slot p of x, for every link between two fluid cells
+// that is not a wall link.  Exchanging the two restores the two-lattice layout (and
+// exchanging again undoes it); cell x exchanges its odd directions p (opp(p) = p + 1), so
+// every pair is exchanged once.  Links across a wall or to a solid cell already hold their
+// value in the cell's own slot p.
+template <typename S>
+__global__ void aa_swap19_kernel(uint32_t N, int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
+                                 const uint16_t *__restrict__ flag, size_t FS, S *f)
+{
+    const uint32_t icu = blockIdx.x * blockDim.x + threadIdx.x;
+    if (icu >= N) return;
+    const uint32_t fl = flag[icu];
+    if (fl & F_SOLID) return;
+    const uint32_t t = fdiv(icu, dnx);
+    const int i = (int)(icu - t * (uint32_t)nx);
+    const uint32_t kk = fdiv(t, dny);
+    const int j = (int)(t - kk * (uint32_t)ny);
+    const int k = (int)kk;
+    const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
+    const long sy = nx, sz = (nz == 1) ? 0 : (long)nx * ny;
+#pragma unroll
+    for (int p = 1; p < 19; p += 2) {
+        const int po = p + 1;
+        // the cell x + c_p: the pull source of direction opp(p)
+        const long nb = (fl & F_FACE) ? face_upwind(po, i, j, k, nx, ny, nz, b1, b2, b3)
+                                      : (long)icu + L19::cx[p] + L19::cy[p] * sy + L19::cz[p] * sz;
+        if (nb < 0 || (flag[nb] & F_SOLID)) continue;
+        S *x = f + (size_t)p * FS + icu;
+        S *y = f + (size_t)po * FS + nb;
+        const S tx = *x;
+        *x = *y;
+        *y = tx;
+    }
 }
 
 }  // namespace lbffs

@@ -47,6 +47,10 @@ enum : int {
     PACK_2D_ZCODE = 8,         // nz==1 with a z-face condition that samples the layer k=1
     PACK_BAD_BOUNDARY = 16,    // boundary value other than 0/1
     PACK_NEISOLID_FACE = 32,   // D3Q27: neiSolid on a wall face (the reference reads solid[-1] there)
+    PACK_AA_FACE = 64,         // in-place streaming: a face code other than Cyclic/BounceBack,
+                               // or different codes on the two sides of an axis
+    PACK_AA_NEISOLID = 128,    // in-place streaming: an interior fluid cell next to a solid one
+                               // without neiSolid (its pulls from the solid cell have no place)
 };
 
 }  // namespace
@@ -57,8 +61,9 @@ struct lbffs_ctx {
     size_t fstride = 0;  // elements between two population arrays on the device (>= N)
     int real_size = 4;   // bytes of the host/compute real type
     int store_size = 4;  // bytes of one stored population
-    void *f[2] = {nullptr, nullptr};
+    void *f[2] = {nullptr, nullptr};  // one array (f[0]) with in-place streaming
     int cur = 0;
+    int aa_odd = 0;  // in-place streaming: the array is in the layout of a first half-step
     int32_t *boundary = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
     uint8_t *solid = nullptr, *neiSolid = nullptr;
     void *sdf = nullptr, *u0 = nullptr, *v0 = nullptr, *w0 = nullptr, *omega = nullptr;
@@ -104,7 +109,7 @@ template <typename Real>
 __global__ void pack_flags_kernel(int Q, uint32_t N, int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
                                   const int32_t *boundary, const int32_t *b1, const int32_t *b2, const int32_t *b3,
                                   const uint8_t *solid, const uint8_t *neiSolid, const Real *sdf, const Real *omega,
-                                  Real omega0, const Real *w0, uint16_t *flag, int *err)
+                                  Real omega0, const Real *w0, int aa, uint16_t *flag, int *err)
 {
     uint32_t ic = blockIdx.x * blockDim.x + threadIdx.x;
     if (ic >= N) return;
@@ -130,6 +135,28 @@ __global__ void pack_flags_kernel(int Q, uint32_t N, int nx, int ny, int nz, Fas
             if (nz == 1 && (c3 == BC_OUTLET || c3 == BC_FIXEDDENSITY || (c3 == BC_FIXEDVELOCITY && w0[ic] < Real(0))))
                 e |= PACK_2D_ZCODE;
         }
+    }
+    if (aa && !sol && bnd == 1) {
+        // in-place streaming classifies a link from both of its cells: the faces of an axis
+        // must agree, and only pulls that wrap or bounce back exist
+        const bool okc = (c1 == BC_CYCLIC || c1 == BC_BOUNCEBACK) && (c2 == BC_CYCLIC || c2 == BC_BOUNCEBACK) &&
+                         (c3 == BC_CYCLIC || c3 == BC_BOUNCEBACK);
+        const long row = (long)ic - i, col = (long)i + (long)nx * ((long)ny * k);
+        const long pl = (long)i + (long)nx * j;
+        bool sym = true;
+        if (i == 0 || i == nx - 1) sym = sym && b1[row + (nx - 1 - i)] == c1;
+        if (j == 0 || j == ny - 1) sym = sym && b2[col + (long)nx * (ny - 1 - j)] == c2;
+        if (nz > 1 && (k == 0 || k == nz - 1)) sym = sym && b3[pl + (long)nx * ny * (nz - 1 - k)] == c3;
+        if (!okc || !sym) e |= PACK_AA_FACE;
+    }
+    if (aa && !sol && bnd != 1 && neiSolid[ic] != 1) {
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int ii = i + dx, jj = j + dy, kz = (nz == 1) ? k : k + dz;
+                    if (ii < 0 || ii >= nx || jj < 0 || jj >= ny || kz < 0 || kz >= nz) continue;
+                    if (solid[(long)ii + (long)nx * ((long)jj + (long)ny * kz)]) e |= PACK_AA_NEISOLID;
+                }
     }
     if (neiSolid[ic] == 1) fl |= F_NEISOLID;
     bool keep = (i <= 1 || i >= nx - 2 || j <= 1 || j >= ny - 2 || (nz > 1 && (k <= 1 || k >= nz - 2)));
@@ -373,7 +400,7 @@ int pack_flags(lbffs_ctx *c)
     hipLaunchKernelGGL((pack_flags_kernel<Real>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream, c->d.q,
                        (uint32_t)c->N, nx, ny, nz, make_fastdiv(nx), make_fastdiv(ny), c->boundary, c->b1, c->b2,
                        c->b3, c->solid, c->neiSolid, (const Real *)c->sdf, (const Real *)c->omega, om0,
-                       (const Real *)c->w0, c->flag, c->d_err);
+                       (const Real *)c->w0, c->d.streaming == LBFFS_STREAM_AA ? 1 : 0, c->flag, c->d_err);
     HIP_TRY(hipGetLastError());
     int err = 0;
     HIP_TRY(hipMemcpyAsync(&err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -400,6 +427,9 @@ int pack_flags(lbffs_ctx *c)
         if (err & PACK_WALLFN_ZMAX) m += " WallFunction on the nzMax face (the reference samples k+1 there);";
         if (err & PACK_2D_ZCODE) m += " nz==1 with an Outlet/FixedDensity/inflowing FixedVelocity z face;";
         if (err & PACK_NEISOLID_FACE) m += " neiSolid on a wall face (D3Q27 reads outside the lattice there);";
+        if (err & PACK_AA_FACE)
+            m += " in-place streaming needs Cyclic or BounceBack faces with the same code on both sides of an axis;";
+        if (err & PACK_AA_NEISOLID) m += " in-place streaming needs neiSolid on interior fluid cells next to a solid cell;";
         return fail(LBFFS_E_INVALID, m);
     }
     c->sponge_mask = 0;
@@ -459,17 +489,59 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
     a.tune = c->tune;
 }
 
+// aa: 0 two-lattice streaming, 1 or 2 the first or second in-place half-step (D3Q19)
 template <typename S, typename Real, int Q, int SGS>
-void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream_t st)
+void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream_t st, int aa)
 {
     if constexpr (Q == 27) {
         hipLaunchKernelGGL((step27_kernel<S, Real, SGS>), dim3(grid), dim3(block), 0, st, a);
     } else {
-        if (a.ibb)
+        if (aa == 1)
+            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 1>), dim3(grid), dim3(block), 0, st, a);
+        else if (aa == 2)
+            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 2>), dim3(grid), dim3(block), 0, st, a);
+        else if (a.ibb)
             hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true>), dim3(grid), dim3(block), 0, st, a);
         else
             hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false>), dim3(grid), dim3(block), 0, st, a);
     }
+}
+
+// In-place streaming: bring the array back to the two-lattice layout (after an odd number
+// of half-steps) before anything reads it by cell; the flag array is the one the last step
+// ran with
+template <typename S>
+int aa_restore_t(lbffs_ctx *c)
+{
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    hipLaunchKernelGGL((aa_swap19_kernel<S>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream, (uint32_t)c->N, nx,
+                       ny, nz, make_fastdiv(nx), make_fastdiv(ny), c->flag, c->fstride, (S *)c->f[0]);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int aa_restore(lbffs_ctx *c)
+{
+    if (c->d.streaming != LBFFS_STREAM_AA || !c->aa_odd) return 0;
+    int e = 0;
+    switch (c->d.storage) {
+    case LBFFS_STORE_F32: e = aa_restore_t<float>(c); break;
+    case LBFFS_STORE_F16DEV: e = aa_restore_t<__half>(c); break;
+    case LBFFS_STORE_BF16DEV: e = aa_restore_t<bf16_t>(c); break;
+    default: e = aa_restore_t<double>(c); break;
+    }
+    if (!e) c->aa_odd = 0;
+    return e;
+}
+
+bool ibm_active(const lbffs_ctx *c);
+
+int aa_step_ok(const lbffs_ctx *c, const lbffs_step_params *p)
+{
+    if (c->d.streaming != LBFFS_STREAM_AA) return 0;
+    if (p->ibb) return fail(LBFFS_E_UNSUPPORTED, "in-place streaming runs without the interpolated bounce-back");
+    if (ibm_active(c)) return fail(LBFFS_E_UNSUPPORTED, "in-place streaming runs without moving-wall markers");
+    return 0;
 }
 
 template <typename Real>
@@ -539,10 +611,11 @@ int run_steps_q(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
     if (ibm) {
         if (int e = ibm_clear_g<Real>(c)) return e;
     }
+    const bool aa = c->d.streaming == LBFFS_STREAM_AA;
     for (int s = 0; s < nsteps; s++) {
         const bool last = (s == nsteps - 1);
         a.fsrc = c->f[c->cur];
-        a.fdst = c->f[1 - c->cur];
+        a.fdst = aa ? c->f[c->cur] : c->f[1 - c->cur];
         a.rho_o = (const Real *)c->rho[c->mcur];
         a.u_o = (const Real *)c->u[c->mcur];
         a.v_o = (const Real *)c->v[c->mcur];
@@ -556,13 +629,17 @@ int run_steps_q(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
         // read the velocities of the marker boxes and the density of the cells they force
         a.mac_all = (a.aux_all || c->d.sgs == LBFFS_SGS_WALE || ibm) ? 1 : 0;
         a.rho_all = (a.aux_all || ibm) ? 1 : 0;
+        const int phase = aa ? 1 + c->aa_odd : 0;
         switch (c->d.sgs) {
-        case LBFFS_SGS_WALE: launch_step<S, Real, Q, 0>(a, grid, block, c->stream); break;
-        case LBFFS_SGS_SMAGORINSKY: launch_step<S, Real, Q, 1>(a, grid, block, c->stream); break;
-        default: launch_step<S, Real, Q, 2>(a, grid, block, c->stream); break;
+        case LBFFS_SGS_WALE: launch_step<S, Real, Q, 0>(a, grid, block, c->stream, phase); break;
+        case LBFFS_SGS_SMAGORINSKY: launch_step<S, Real, Q, 1>(a, grid, block, c->stream, phase); break;
+        default: launch_step<S, Real, Q, 2>(a, grid, block, c->stream, phase); break;
         }
         HIP_TRY(hipGetLastError());
-        c->cur ^= 1;
+        if (aa)
+            c->aa_odd ^= 1;
+        else
+            c->cur ^= 1;
         c->mcur ^= 1;
         if (ibm) {
             if (int e = ibm_after_step<S, Real, Q>(c, last)) return e;
@@ -584,8 +661,10 @@ int upload_f(lbffs_ctx *c, const void *host)
     if (sizeof(S) == sizeof(Real)) {
         // host q-major arrays of N -> device arrays of stride fstride
         for (int b = 0; b < 2; b++)
-            HIP_TRY(hipMemcpy2D(c->f[b], c->fstride * sizeof(Real), host, c->N * sizeof(Real), c->N * sizeof(Real),
-                                c->d.q, hipMemcpyHostToDevice));
+            if (c->f[b])
+                HIP_TRY(hipMemcpy2D(c->f[b], c->fstride * sizeof(Real), host, c->N * sizeof(Real),
+                                    c->N * sizeof(Real), c->d.q, hipMemcpyHostToDevice));
+        c->aa_odd = 0;
         return 0;
     }
     void *tmp = nullptr;
@@ -593,6 +672,7 @@ int upload_f(lbffs_ctx *c, const void *host)
     hipError_t e1 = hipMemcpy(tmp, host, n * sizeof(Real), hipMemcpyHostToDevice);
     if (e1 == hipSuccess) {
         for (int b = 0; b < 2; b++) {
+            if (!c->f[b]) continue;
             if (c->d.q == 27)
                 hipLaunchKernelGGL((encode_kernel<S, Real, 27>), dim3(grid_for(c->N, 256)), dim3(256), 0, c->stream,
                                    (uint32_t)c->N, c->fstride, (const Real *)tmp, (S *)c->f[b]);
@@ -605,6 +685,7 @@ int upload_f(lbffs_ctx *c, const void *host)
     }
     hipFree(tmp);
     if (e1 != hipSuccess) return fail(LBFFS_E_DEVICE, std::string("upload f: ") + hipGetErrorString(e1));
+    c->aa_odd = 0;
     return 0;
 }
 
@@ -898,6 +979,10 @@ int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out)
     if (desc->sgs < 0 || desc->sgs > 2) return fail(LBFFS_E_INVALID, "sgs");
     if (desc->rounding != LBFFS_ROUND_STOCHASTIC && desc->rounding != LBFFS_ROUND_NEAREST)
         return fail(LBFFS_E_INVALID, "rounding");
+    if (desc->streaming != LBFFS_STREAM_TWO_LATTICE && desc->streaming != LBFFS_STREAM_AA)
+        return fail(LBFFS_E_INVALID, "streaming");
+    if (desc->streaming == LBFFS_STREAM_AA && desc->q != 19)
+        return fail(LBFFS_E_UNSUPPORTED, "in-place streaming runs D3Q19 lattices");
     if (desc->q == 19 && desc->collision != LBFFS_COLL_RR)
         return fail(LBFFS_E_UNSUPPORTED, "D3Q19 runs the recursive-regularized collision (K19:3637)");
     if (desc->q == 27 && desc->collision != LBFFS_COLL_CUMULANT)
@@ -930,7 +1015,7 @@ int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out)
     }
     c->fstride = population_stride(n, c->store_size);
     if (!e) e = dalloc(&c->f[0], (size_t)desc->q * c->fstride * c->store_size);
-    if (!e) e = dalloc(&c->f[1], (size_t)desc->q * c->fstride * c->store_size);
+    if (!e && desc->streaming != LBFFS_STREAM_AA) e = dalloc(&c->f[1], (size_t)desc->q * c->fstride * c->store_size);
     if (!e) e = dalloc((void **)&c->boundary, n * 4);
     if (!e) e = dalloc((void **)&c->b1, n * 4);
     if (!e) e = dalloc((void **)&c->b2, n * 4);
@@ -1004,7 +1089,9 @@ int lbffs_step(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
     if (nsteps < 0) return fail(LBFFS_E_INVALID, "nsteps < 0");
     if (!c->f_loaded) return fail(LBFFS_E_STATE, "populations were never uploaded");
     if (p->ibb && !c->qlink) return fail(LBFFS_E_STATE, "ibb = 1 needs the q-distances (LBFFS_FIELD_QLINK)");
+    if (int e = aa_step_ok(c, p)) return e;
     if (c->flags_dirty) {
+        if (int e = aa_restore(c)) return e;  // with the flags the array was streamed with
         int e = is_f64(c) ? pack_flags<double>(c) : pack_flags<float>(c);
         if (e) return e;
     }
@@ -1024,10 +1111,13 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
     if (trial_steps < 1) return fail(LBFFS_E_INVALID, "trial_steps < 1");
     if (!c->f_loaded) return fail(LBFFS_E_STATE, "populations were never uploaded");
     if (p->ibb && !c->qlink) return fail(LBFFS_E_STATE, "ibb = 1 needs the q-distances (LBFFS_FIELD_QLINK)");
+    if (int e = aa_step_ok(c, p)) return e;
     if (c->flags_dirty) {
+        if (int e = aa_restore(c)) return e;
         int e = is_f64(c) ? pack_flags<double>(c) : pack_flags<float>(c);
         if (e) return e;
     }
+    if (int e = aa_restore(c)) return e;  // the saved array then restarts at a first half-step
     HIP_TRY(hipStreamSynchronize(c->stream));
     // save everything a step writes, so that tuning leaves the lattice state unchanged
     const size_t n = (size_t)c->N, rs = c->real_size;
@@ -1084,6 +1174,7 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
     // restore the state
     c->cur = cur0;
     c->mcur = mcur0;
+    c->aa_odd = 0;
     if (he == hipSuccess) he = hipMemcpyAsync(c->f[cur0], saved_f, fbytes, hipMemcpyDeviceToDevice, c->stream);
     for (int k = 0; k < 11 && he == hipSuccess; k++)
         he = hipMemcpyAsync(keep[k], (char *)saved_r + k * n * rs, n * rs, hipMemcpyDeviceToDevice, c->stream);
@@ -1180,7 +1271,10 @@ int lbffs_download(lbffs_ctx *c, int field, void *host, size_t bytes)
         return fail(LBFFS_E_INVALID, "download size " + std::to_string(bytes) + " != " +
                                          std::to_string(field_host_bytes(c, field)));
     if (field == LBFFS_FIELD_QLINK && !c->qlink) return fail(LBFFS_E_STATE, "q-distances were never uploaded");
-    if (field == LBFFS_FIELD_F) return dispatch<DownloadFOp>(c, host);
+    if (field == LBFFS_FIELD_F) {
+        if (int e = aa_restore(c)) return e;
+        return dispatch<DownloadFOp>(c, host);
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(host, field_ptr(c, field), bytes, hipMemcpyDeviceToHost));
     return 0;
@@ -1190,9 +1284,11 @@ int lbffs_reduce(lbffs_ctx *c, double out[LBFFS_REDUCE_N])
 {
     if (!c || !out) return fail(LBFFS_E_INVALID, "null argument");
     if (c->flags_dirty) {
+        if (int e = aa_restore(c)) return e;
         int e = is_f64(c) ? pack_flags<double>(c) : pack_flags<float>(c);
         if (e) return e;
     }
+    if (int e = aa_restore(c)) return e;  // per-cell density sums in the two-lattice layout
     return dispatch<ReduceOp>(c, out);
 }
 
@@ -1203,6 +1299,7 @@ int lbffs_output_macros(lbffs_ctx *c, float dpdx, void *rho, void *u, void *v, v
     if (bytes != (size_t)c->N * (size_t)c->real_size)
         return fail(LBFFS_E_INVALID, "output_macros size " + std::to_string(bytes) + " != " +
                                          std::to_string((size_t)c->N * (size_t)c->real_size));
+    if (int e = aa_restore(c)) return e;
     HIP_TRY(hipStreamSynchronize(c->stream));
     void *const h[4] = {rho, u, v, w};
     return dispatch<OutputMacrosOp>(c, (double)dpdx, h);
@@ -1257,6 +1354,9 @@ int lbffs_batch_create(const lbffs_desc *descs, int n, lbffs_batch **out)
         if (descs[d].q != descs[0].q || descs[d].storage != descs[0].storage || descs[d].sgs != descs[0].sgs ||
             descs[d].collision != descs[0].collision)
             return fail(LBFFS_E_INVALID, "the lattices of a batch share q, storage, sgs and collision");
+    for (int d = 0; d < n; d++)
+        if (descs[d].streaming != LBFFS_STREAM_TWO_LATTICE)
+            return fail(LBFFS_E_UNSUPPORTED, "the lattices of a batch stream between two arrays");
     lbffs_batch *b = new lbffs_batch();
     hipError_t he = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (he == hipSuccess) he = hipEventCreate(&b->ev0);

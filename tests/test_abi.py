@@ -83,3 +83,18 @@ def test_missing_library_raises(monkeypatch):
     monkeypatch.setattr(lb, "LIB_PATH", "/nonexistent/liblbffs.so")
     with pytest.raises(lb.LbffsError):
         lb.lib()
+
+
+def test_streaming_descriptor_checks():
+    # in-place streaming: validated at create before any device call
+    L = lb.lib()
+    h = ctypes.c_void_p()
+    d = lb.Desc(abi_version=lb.ABI_VERSION, q=19, nx=8, ny=8, nz=8, omega0=1.0, streaming=7)
+    assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"streaming" in L.lbffs_last_error()
+    d = lb.Desc(abi_version=lb.ABI_VERSION, q=27, nx=8, ny=8, nz=8, omega0=1.0, collision=lb.COLL_CUMULANT,
+                streaming=lb.STREAM_AA)
+    assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -4
+    descs = (lb.Desc * 1)(lb.Desc(abi_version=lb.ABI_VERSION, q=19, nx=8, ny=8, nz=8, omega0=1.0,
+                                  streaming=lb.STREAM_AA))
+    assert L.lbffs_batch_create(descs, 1, ctypes.byref(h)) == -4
