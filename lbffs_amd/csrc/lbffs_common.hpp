@@ -141,6 +141,12 @@ struct Codec<float, float, Q> {
     __device__ static inline void store(float *p, float v, int, float) { *p = v; }
     __device__ static inline void store_nt(float *p, float v, int, float) { __builtin_nontemporal_store(v, p); }
     __device__ static inline void store_nt_sr(float *p, float v, int q, float w, uint32_t) { store_nt(p, v, q, w); }
+    __device__ static inline void store_nt_sr2(float *p0, float *p1, float v0, float v1, int q0, int q1, float w0,
+                                               float w1, uint32_t, uint32_t)
+    {
+        store_nt(p0, v0, q0, w0);
+        store_nt(p1, v1, q1, w1);
+    }
 };
 
 template <int Q>
@@ -149,6 +155,12 @@ struct Codec<double, double, Q> {
     __device__ static inline void store(double *p, double v, int, double) { *p = v; }
     __device__ static inline void store_nt(double *p, double v, int, double) { __builtin_nontemporal_store(v, p); }
     __device__ static inline void store_nt_sr(double *p, double v, int q, double w, uint32_t) { store_nt(p, v, q, w); }
+    __device__ static inline void store_nt_sr2(double *p0, double *p1, double v0, double v1, int q0, int q1, double w0,
+                                               double w1, uint32_t, uint32_t)
+    {
+        store_nt(p0, v0, q0, w0);
+        store_nt(p1, v1, q1, w1);
+    }
 };
 
 template <int Q>
@@ -176,6 +188,22 @@ struct Codec<__half, float, Q> {
         __builtin_memcpy(&bits, &e, 2);
         __builtin_nontemporal_store(bits, (unsigned short *)p);
     }
+    // two populations rounded into the two halves of one register (no register to clear
+    // for the conversion's other half), stored from its low and high half
+    __device__ static inline void store_nt_sr2(__half *p0, __half *p1, float v0, float v1, int q0, int q1, float w0,
+                                               float w1, uint32_t r0, uint32_t r1)
+    {
+        typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+        h2_t h = {0, 0};
+        h = __builtin_amdgcn_cvt_sr_f16_f32(h, (v0 - w0) * DevScale<Q>::scale(q0), r0, false);
+        h = __builtin_amdgcn_cvt_sr_f16_f32(h, (v1 - w1) * DevScale<Q>::scale(q1), r1, true);
+        const _Float16 e0 = h[0], e1 = h[1];
+        unsigned short b0, b1;
+        __builtin_memcpy(&b0, &e0, 2);
+        __builtin_memcpy(&b1, &e1, 2);
+        __builtin_nontemporal_store(b0, (unsigned short *)p0);
+        __builtin_nontemporal_store(b1, (unsigned short *)p1);
+    }
 };
 
 template <int Q>
@@ -198,6 +226,16 @@ struct Codec<bf16_t, float, Q> {
         b2_t h = {0, 0};
         h = __builtin_amdgcn_cvt_sr_bf16_f32(h, (v - w) * DevScale<Q>::scale(q), r, false);
         __builtin_nontemporal_store((unsigned short)h[0], (unsigned short *)p);
+    }
+    __device__ static inline void store_nt_sr2(bf16_t *p0, bf16_t *p1, float v0, float v1, int q0, int q1, float w0,
+                                               float w1, uint32_t r0, uint32_t r1)
+    {
+        typedef short b2_t __attribute__((ext_vector_type(2)));
+        b2_t h = {0, 0};
+        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, (v0 - w0) * DevScale<Q>::scale(q0), r0, false);
+        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, (v1 - w1) * DevScale<Q>::scale(q1), r1, true);
+        __builtin_nontemporal_store((unsigned short)h[0], (unsigned short *)p0);
+        __builtin_nontemporal_store((unsigned short)h[1], (unsigned short *)p1);
     }
 };
 
