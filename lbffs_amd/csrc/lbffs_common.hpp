@@ -93,6 +93,16 @@ struct DevScale<27> {
     }
 };
 
+// (v - w) scale(q) as one multiply-add: the scale is a power of two, so v scale and
+// w scale are exact and fma(v, s, -w s) rounds s (v - w) once -- the same float as the
+// subtraction followed by the (exact) scaling, in one instruction
+template <int Q>
+__device__ inline float dev_scaled(float v, float w, int q)
+{
+    const float s = DevScale<Q>::scale(q);
+    return __builtin_fmaf(v, s, -(w * s));
+}
+
 struct bf16_t {
     uint16_t bits;
 };
@@ -171,18 +181,18 @@ struct Codec<__half, float, Q> {
     }
     __device__ static inline void store(__half *p, float v, int q, float w)
     {
-        *p = __float2half_rn((v - w) * DevScale<Q>::scale(q));
+        *p = __float2half_rn(dev_scaled<Q>(v, w, q));
     }
     __device__ static inline void store_nt(__half *p, float v, int q, float w)
     {
-        __half h = __float2half_rn((v - w) * DevScale<Q>::scale(q));
+        __half h = __float2half_rn(dev_scaled<Q>(v, w, q));
         __builtin_nontemporal_store(__half_as_ushort(h), (unsigned short *)p);
     }
     __device__ static inline void store_nt_sr(__half *p, float v, int q, float w, uint32_t r)
     {
         typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
         h2_t h = {0, 0};
-        h = __builtin_amdgcn_cvt_sr_f16_f32(h, (v - w) * DevScale<Q>::scale(q), r, false);
+        h = __builtin_amdgcn_cvt_sr_f16_f32(h, dev_scaled<Q>(v, w, q), r, false);
         const _Float16 e = h[0];
         unsigned short bits;
         __builtin_memcpy(&bits, &e, 2);
@@ -195,8 +205,8 @@ struct Codec<__half, float, Q> {
     {
         typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
         h2_t h = {0, 0};
-        h = __builtin_amdgcn_cvt_sr_f16_f32(h, (v0 - w0) * DevScale<Q>::scale(q0), r0, false);
-        h = __builtin_amdgcn_cvt_sr_f16_f32(h, (v1 - w1) * DevScale<Q>::scale(q1), r1, true);
+        h = __builtin_amdgcn_cvt_sr_f16_f32(h, dev_scaled<Q>(v0, w0, q0), r0, false);
+        h = __builtin_amdgcn_cvt_sr_f16_f32(h, dev_scaled<Q>(v1, w1, q1), r1, true);
         const _Float16 e0 = h[0], e1 = h[1];
         unsigned short b0, b1;
         __builtin_memcpy(&b0, &e0, 2);
@@ -214,17 +224,17 @@ struct Codec<bf16_t, float, Q> {
     }
     __device__ static inline void store(bf16_t *p, float v, int q, float w)
     {
-        p->bits = float_to_bf16((v - w) * DevScale<Q>::scale(q));
+        p->bits = float_to_bf16(dev_scaled<Q>(v, w, q));
     }
     __device__ static inline void store_nt(bf16_t *p, float v, int q, float w)
     {
-        __builtin_nontemporal_store(float_to_bf16((v - w) * DevScale<Q>::scale(q)), (unsigned short *)p);
+        __builtin_nontemporal_store(float_to_bf16(dev_scaled<Q>(v, w, q)), (unsigned short *)p);
     }
     __device__ static inline void store_nt_sr(bf16_t *p, float v, int q, float w, uint32_t r)
     {
         typedef short b2_t __attribute__((ext_vector_type(2)));
         b2_t h = {0, 0};
-        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, (v - w) * DevScale<Q>::scale(q), r, false);
+        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, dev_scaled<Q>(v, w, q), r, false);
         __builtin_nontemporal_store((unsigned short)h[0], (unsigned short *)p);
     }
     __device__ static inline void store_nt_sr2(bf16_t *p0, bf16_t *p1, float v0, float v1, int q0, int q1, float w0,
@@ -232,8 +242,8 @@ struct Codec<bf16_t, float, Q> {
     {
         typedef short b2_t __attribute__((ext_vector_type(2)));
         b2_t h = {0, 0};
-        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, (v0 - w0) * DevScale<Q>::scale(q0), r0, false);
-        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, (v1 - w1) * DevScale<Q>::scale(q1), r1, true);
+        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, dev_scaled<Q>(v0, w0, q0), r0, false);
+        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, dev_scaled<Q>(v1, w1, q1), r1, true);
         __builtin_nontemporal_store((unsigned short)h[0], (unsigned short *)p0);
         __builtin_nontemporal_store((unsigned short)h[1], (unsigned short *)p1);
     }
