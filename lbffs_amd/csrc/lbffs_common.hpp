@@ -148,6 +148,7 @@ struct Codec;
 template <int Q>
 struct Codec<float, float, Q> {
     __device__ static inline float load(const float *p, int, float) { return *p; }
+    __device__ static inline float load_nt(const float *p, int, float) { return __builtin_nontemporal_load(p); }
     __device__ static inline void store(float *p, float v, int, float) { *p = v; }
     __device__ static inline void store_nt(float *p, float v, int, float) { __builtin_nontemporal_store(v, p); }
     __device__ static inline void store_nt_sr(float *p, float v, int q, float w, uint32_t) { store_nt(p, v, q, w); }
@@ -162,6 +163,7 @@ struct Codec<float, float, Q> {
 template <int Q>
 struct Codec<double, double, Q> {
     __device__ static inline double load(const double *p, int, double) { return *p; }
+    __device__ static inline double load_nt(const double *p, int, double) { return __builtin_nontemporal_load(p); }
     __device__ static inline void store(double *p, double v, int, double) { *p = v; }
     __device__ static inline void store_nt(double *p, double v, int, double) { __builtin_nontemporal_store(v, p); }
     __device__ static inline void store_nt_sr(double *p, double v, int q, double w, uint32_t) { store_nt(p, v, q, w); }
@@ -178,6 +180,11 @@ struct Codec<__half, float, Q> {
     __device__ static inline float load(const __half *p, int q, float w)
     {
         return w + __half2float(*p) * DevScale<Q>::inv(q);
+    }
+    __device__ static inline float load_nt(const __half *p, int q, float w)
+    {
+        const unsigned short b = __builtin_nontemporal_load((const unsigned short *)p);
+        return w + __half2float(__ushort_as_half(b)) * DevScale<Q>::inv(q);
     }
     __device__ static inline void store(__half *p, float v, int q, float w)
     {
@@ -221,6 +228,10 @@ struct Codec<bf16_t, float, Q> {
     __device__ static inline float load(const bf16_t *p, int q, float w)
     {
         return w + bf16_to_float(p->bits) * DevScale<Q>::inv(q);
+    }
+    __device__ static inline float load_nt(const bf16_t *p, int q, float w)
+    {
+        return w + bf16_to_float(__builtin_nontemporal_load(&p->bits)) * DevScale<Q>::inv(q);
     }
     __device__ static inline void store(bf16_t *p, float v, int q, float w)
     {
