@@ -62,6 +62,21 @@ int lbffs_case_neisolid(int q, int nx, int ny, int nz, const uint8_t *solid, uin
 int lbffs_case_outlet_sponge(int nx, int ny, int nz, int i, int j, int k, int p1, int p2, int p3, int spz,
                              float omega0, float *omega);
 
+/* stl.hpp:317-453 STLpatch::findPoints: the lattice face cells an inlet or outlet patch
+ * covers, for the patch loops of ns.cpp:278-377 (which then set the cells' face codes and
+ * u0/v0/w0, and for outlets call lbffs_case_outlet_sponge per cell).  Per triangle, in
+ * order: its plane is the axis of the largest |normal| component -- |.| is the reference's
+ * unqualified abs() on a float, which resolves to int abs(int) with the headers stl.hpp
+ * sees (a component is truncated toward zero first) -- and if its centroid truncates to 0
+ * or n-1 of that axis (or to -1, n-2 with the +1 of stl.hpp), every cell of that lattice
+ * face whose in-plane coordinates lie in the triangle (pointInElement, stl.hpp:95-111, FP32,
+ * bounds inclusive) is listed; a cell in several triangles is listed once per triangle.
+ * v0, v1, v2, normal, centroid: rows x, y, z of n floats each (3*n), lattice units.  Writes
+ * at most cap indices ic = i + nx*(j + ny*k) to points (may be NULL with cap = 0) and
+ * the full count to *count. */
+int lbffs_case_find_points(int n, const float *v0, const float *v1, const float *v2, const float *normal,
+                           const float *centroid, int nx, int ny, int nz, int32_t *points, long cap, long *count);
+
 /* walls.hpp:372-469 setSDFnearBwalls: sdf = 0.5 + d in the int(dr)+1 layers d = 0, 1, ...
  * next to every domain face whose face cell has boundary code 1 (BounceBack) in the
  * face's normal-direction code array (b1 for x faces, b2 for y, b3 for z); a layer that

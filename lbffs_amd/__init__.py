@@ -102,6 +102,7 @@ def lib():
         L.lbffs_case_init_feq.argtypes = [ip, ip, ip, ip, C.c_float, C.c_float, C.c_float, C.c_float, vp]
         L.lbffs_case_neisolid.argtypes = [ip, ip, ip, ip, vp, vp]
         L.lbffs_case_outlet_sponge.argtypes = [ip] * 10 + [C.c_float, vp]
+        L.lbffs_case_find_points.argtypes = [ip, vp, vp, vp, vp, vp, ip, ip, ip, vp, C.c_long, C.POINTER(C.c_long)]
         L.lbffs_probe_copy.argtypes = [sz, ip, C.POINTER(C.c_float)]
         L.lbffs_probe_bandwidth.argtypes = [ip, sz, ip, C.POINTER(C.c_float)]
         L.lbffs_autotune.argtypes = [vp, C.POINTER(StepParams), ip, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -132,6 +133,7 @@ def lib():
         for n in ("lbffs_create", "lbffs_upload", "lbffs_download", "lbffs_step", "lbffs_reduce",
                   "lbffs_synchronize", "lbffs_last_step_ms", "lbffs_destroy", "lbffs_case_units",
                   "lbffs_case_faces", "lbffs_case_init_feq", "lbffs_case_neisolid", "lbffs_case_outlet_sponge",
+                  "lbffs_case_find_points",
                   "lbffs_probe_copy", "lbffs_probe_bandwidth", "lbffs_autotune", "lbffs_set_launch",
                   "lbffs_ibm_set_markers", "lbffs_ibm_set_motion", "lbffs_ibm_download", "lbffs_batch_create",
                   "lbffs_batch_context", "lbffs_batch_set_launch", "lbffs_batch_step", "lbffs_batch_last_step_ms",

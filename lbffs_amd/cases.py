@@ -149,6 +149,30 @@ def channel(nx=8, ny=20, nz=1, tau=0.8, force=1e-5, sgs=SGS_NONE, q=19):
     return make_case("channel", nx, ny, nz, 1.0, 1.0, U0, nu, codes, dpdx=dpdx, sgs=sgs, q=q)
 
 
+def apply_patch(case, points, codes, vel=(0.0, 0.0, 0.0), spz=None):
+    """The patch loops of ns.cpp:278-377 for one inlet or outlet patch: the listed face
+    cells (geometry.find_points) get the patch's face codes (names or numbers, per axis)
+    and velocity (lattice units, as inputBoundaryCondition returns it after dividing by c);
+    an outlet patch with spz cells gets the cosine omega sponge around each cell
+    (lbffs_case_outlet_sponge, whose tau0 is the 0.55 of ns.cpp:327).  Updates case.arrays
+    in place."""
+    a = case.arrays
+    p = [BC[cd] if isinstance(cd, str) else int(cd) for cd in codes]
+    pts = np.asarray(points, np.int64)
+    for d, key in enumerate(("boundary1", "boundary2", "boundary3")):
+        a[key][pts] = p[d]
+    for d, key in enumerate(("u0", "v0", "w0")):
+        a[key][pts] = np.float32(vel[d])
+    if spz is not None:
+        nx, ny = case.nx, case.ny
+        om = a["omega"]
+        for ic in pts.tolist():
+            i, j, k = ic % nx, (ic // nx) % ny, ic // (nx * ny)
+            _check(lib().lbffs_case_outlet_sponge(case.nx, case.ny, case.nz, i, j, k, p[0], p[1], p[2], int(spz),
+                                                  float(case.omega0), _ptr(om)), "lbffs_case_outlet_sponge")
+    return case
+
+
 def cylinder_qlinks(q, nx, ny, nz, solid, cx, cy, R):
     """q-distances of the links that leave a fluid cell towards a solid cell of the cylinder
     (axis along z through (cx, cy), radius R), in FP64: the link from the fluid node x to

@@ -248,3 +248,61 @@ int lbffs_case_sdf_near_bwalls(float dr, const int32_t *b1, const int32_t *b2, c
 }
 
 }  // extern "C"
+
+namespace {
+
+// stl.hpp:95-111 pointInElement, in FP32 as written
+bool point_in_element(float x, float y, float x1, float y1, float x2, float y2)
+{
+    const float s = (x * y2 - x2 * y) / (x1 * y2 - x2 * y1);
+    const float t = (x * y1 - x1 * y) / (x2 * y1 - x1 * y2);
+    return (s + t >= 0.f && s + t <= 1.f) && (s >= 0.f && s <= 1.f) && (t >= 0.f && t <= 1.f);
+}
+
+// the reference's abs() of a float component: int abs(int) of the truncated value
+float int_abs(float v) { return (float)std::abs((int)v); }
+
+}  // namespace
+
+extern "C" int lbffs_case_find_points(int n, const float *v0, const float *v1, const float *v2, const float *normal,
+                                      const float *centroid, int nx, int ny, int nz, int32_t *points, long cap,
+                                      long *count)
+{
+    if (n < 0 || nx < 1 || ny < 1 || nz < 1 || !count || cap < 0 || (cap > 0 && !points)) return -1;
+    if (n > 0 && (!v0 || !v1 || !v2 || !normal || !centroid)) return -1;
+    long m = 0;
+    auto push = [&](long ic) {
+        if (m < cap) points[m] = (int32_t)ic;
+        m++;
+    };
+    const int dims[3] = {nx, ny, nz};
+    for (int t = 0; t < n; t++) {
+        const float a0 = int_abs(normal[t]), a1 = int_abs(normal[n + t]), a2 = int_abs(normal[2 * n + t]);
+        const float mx = std::fmax(a0, std::fmax(a1, a2));
+        const int ax = (a0 == mx) ? 0 : ((a1 == mx) ? 1 : 2);
+        // the two in-plane axes, in the reference's loop order
+        const int u = ax == 0 ? 1 : 0, w = ax == 2 ? 1 : 2;
+        const int ci = (int)centroid[ax * n + t];
+        int face;
+        if (ci == 0 || ci + 1 == 0)
+            face = 0;
+        else if (ci == dims[ax] - 1 || ci + 1 == dims[ax] - 1)
+            face = dims[ax] - 1;
+        else
+            continue;
+        const float x1 = v1[u * n + t] - v0[u * n + t], y1 = v1[w * n + t] - v0[w * n + t];
+        const float x2 = v2[u * n + t] - v0[u * n + t], y2 = v2[w * n + t] - v0[w * n + t];
+        for (int a = 0; a < dims[u]; a++)
+            for (int b = 0; b < dims[w]; b++) {
+                const float x = (float)a - v0[u * n + t], y = (float)b - v0[w * n + t];
+                if (!point_in_element(x, y, x1, y1, x2, y2)) continue;
+                int idx[3];
+                idx[ax] = face;
+                idx[u] = a;
+                idx[w] = b;
+                push((long)idx[0] + (long)nx * ((long)idx[1] + (long)ny * idx[2]));
+            }
+    }
+    *count = m;
+    return 0;
+}

@@ -245,6 +245,22 @@ def qlinks(stl: STL, q: int, nx: int, ny: int, nz: int, solid, nei, init: float 
     return ql, miss.value
 
 
+def find_points(patch: Patch, nx: int, ny: int, nz: int) -> np.ndarray:
+    """stl.hpp:317-453 findPoints (lbffs_case_find_points): the lattice face cells the
+    patch's triangles cover, triangle by triangle in the reference's order (a cell inside
+    two triangles appears twice)."""
+    rows = [np.ascontiguousarray(x, np.float32) for x in (patch.v0, patch.v1, patch.v2, patch.normal, patch.c)]
+    n = rows[0].shape[1]
+    cnt = C.c_long()
+    _check(lib().lbffs_case_find_points(n, *[_ptr(r) for r in rows], nx, ny, nz, None, 0, C.byref(cnt)),
+           "lbffs_case_find_points")
+    out = np.empty(cnt.value, np.int32)
+    if cnt.value:
+        _check(lib().lbffs_case_find_points(n, *[_ptr(r) for r in rows], nx, ny, nz, _ptr(out), cnt.value,
+                                            C.byref(cnt)), "lbffs_case_find_points")
+    return out
+
+
 def qlinks_into(lat, stl: STL, init: float = 0.5) -> int:
     """The same q-distances cast straight into the lattice's "qlink" field on the device
     (lbffs_geom_qlinks_ctx; the lattice's uploaded solid and neiSolid flags), without a

@@ -475,3 +475,106 @@ def test_write_stl_round_trip(tmp_path):
     a, b = s.patches[0], r.patches[0]
     for k in ("v0", "v1", "v2", "normal", "c"):
         assert np.array_equal(getattr(a, k), getattr(b, k)), k
+
+
+def _patch(tris, normal):
+    """A patch from a list of triangles (three vertices each) with one normal for all."""
+    v = [np.array([t[k] for t in tris], np.float32).T.copy() for k in range(3)]
+    n = np.repeat(np.array(normal, np.float32)[:, None], len(tris), axis=1)
+    p = geo.Patch("p", v[0], v[1], v[2], normal=n)
+    p.c = ((p.v0 + p.v1) + p.v2) / np.float32(3.0)
+    return p
+
+
+def _in_tri_exact(x, y, x1, y1, x2, y2):
+    # stl.hpp:95-111 pointInElement in exact rational arithmetic
+    from fractions import Fraction as Fr
+    x, y, x1, y1, x2, y2 = (Fr(float(a)) for a in (x, y, x1, y1, x2, y2))
+    d1, d2 = x1 * y2 - x2 * y1, x2 * y1 - x1 * y2
+    if d1 == 0:
+        return False
+    s, t = (x * y2 - x2 * y) / d1, (x * y1 - x1 * y) / d2
+    return 0 <= s + t <= 1 and 0 <= s <= 1 and 0 <= t <= 1
+
+
+def test_find_points_rectangle_on_each_face():
+    # two right triangles (legs 4 cells) spanning a 4 x 4 square on a lattice face: every
+    # lattice point of the closed square is listed, the points of the shared diagonal twice;
+    # s and t are multiples of 1/4 there, exact in FP32, so the exact rule decides
+    nx, ny, nz = 9, 8, 7
+    for ax, face in ((0, 0), (0, nx - 1), (1, 0), (1, ny - 1), (2, 0), (2, nz - 1)):
+        u, w = (1, 2) if ax == 0 else ((0, 2) if ax == 1 else (0, 1))
+        def P(a, b):
+            p = [0.0, 0.0, 0.0]
+            p[ax], p[u], p[w] = float(face), a, b
+            return tuple(p)
+        a0, b0 = 1.0, 2.0
+        tris = [(P(a0, b0), P(a0 + 4, b0), P(a0 + 4, b0 + 4)), (P(a0, b0), P(a0 + 4, b0 + 4), P(a0, b0 + 4))]
+        nrm = [0.0, 0.0, 0.0]
+        nrm[ax] = -1.0 if face == 0 else 1.0
+        pts = geo.find_points(_patch(tris, nrm), nx, ny, nz)
+        dims = (nx, ny, nz)
+        want = []
+        for tri in tris:
+            (pu0, pw0), (pu1, pw1), (pu2, pw2) = ((t[u], t[w]) for t in tri)
+            for a in range(dims[u]):
+                for b in range(dims[w]):
+                    if _in_tri_exact(a - pu0, b - pw0, pu1 - pu0, pw1 - pw0, pu2 - pu0, pw2 - pw0):
+                        idx = [0, 0, 0]
+                        idx[ax], idx[u], idx[w] = face, a, b
+                        want.append(idx[0] + nx * (idx[1] + ny * idx[2]))
+        assert pts.tolist() == want, (ax, face)
+        assert len(set(pts.tolist())) == 25 and len(pts) == 25 + 5  # 5x5 points, diagonal twice
+
+
+def test_find_points_plane_selection():
+    nx, ny, nz = 9, 8, 7
+    # a triangle whose centroid is inside the lattice, not on a face: no points
+    tri = [((4.0, 1.0, 1.0), (4.0, 5.0, 1.0), (4.0, 5.0, 5.0))]
+    assert geo.find_points(_patch(tri, (1.0, 0.0, 0.0)), nx, ny, nz).size == 0
+    # centroid just outside the lattice (x in (-1, 0]): still the i = 0 face
+    tri = [((-0.4, 1.0, 1.0), (-0.4, 5.0, 1.0), (-0.4, 5.0, 5.0))]
+    assert geo.find_points(_patch(tri, (-1.0, 0.0, 0.0)), nx, ny, nz).size > 0
+    # a component below 1 in magnitude truncates to 0 in the reference's abs(): the normal
+    # (0, 0.6, 0.8) selects the x plane (first axis), so a patch on the k = 0 face with that
+    # normal lists nothing
+    tri = [((1.0, 1.0, 0.0), (5.0, 1.0, 0.0), (5.0, 5.0, 0.0))]
+    assert geo.find_points(_patch(tri, (0.0, 0.6, 0.8)), nx, ny, nz).size == 0
+    assert geo.find_points(_patch(tri, (0.0, 0.0, -1.0)), nx, ny, nz).size > 0
+
+
+def test_inlet_patch_reproduces_the_face_setup():
+    # an inlet patch covering the whole nxMin face (two triangles beyond its corners) with
+    # FixedVelocity and the inlet velocity gives the face codes and u0 that the face setup
+    # gives for a FixedVelocity nxMin face; an outlet patch on nxMax with a sponge changes
+    # omega only within spz cells of that face
+    nx, ny, nz = 12, 8, 6
+    ref = cases.cylinder_channel(nx=nx, ny=ny, nz=nz, D=2, cx=6, cy=4, spz_cells=2, noise=0.0)
+    base_codes = {f: ("Cyclic", "Cyclic", "Cyclic") for f in cases.FACES}
+    base_codes["nyMin"] = ("Cyclic", "BounceBack", "Cyclic")
+    base_codes["nyMax"] = ("Cyclic", "BounceBack", "Cyclic")
+    c = cases.make_case("patched", nx, ny, nz, 1.0, 1.0, ref.meta["u_in"], ref.units["nu"] * (nx - 1) * ref.meta["u_in"],
+                        base_codes, q=19)
+    lo, hi = -0.5, max(ny, nz) * 2.0
+    tris = [((0.0, lo, lo), (0.0, hi, lo), (0.0, hi, hi)), ((0.0, lo, lo), (0.0, hi, hi), (0.0, lo, hi))]
+    pts = geo.find_points(_patch(tris, (-1.0, 0.0, 0.0)), nx, ny, nz)
+    assert set(pts.tolist()) == {nx * (j + ny * k) for j in range(ny) for k in range(nz)}
+    # the face setup leaves the y-wall edge cells' x code as it is: patch only the others
+    keep = np.isin(np.arange(nx * ny * nz), pts) & (ref.arrays["boundary1"] == lb.BC["FixedVelocity"])
+    cases.apply_patch(c, np.nonzero(keep)[0], ("FixedVelocity", "Cyclic", "Cyclic"), vel=(ref.meta["u_in"], 0.0, 0.0))
+    i0 = np.arange(nx * ny * nz) % nx == 0
+    assert np.array_equal(c.arrays["boundary1"][i0], ref.arrays["boundary1"][i0])
+    # the face setup gives the velocity to the face's interior cells (not its edges)
+    ic = np.arange(nx * ny * nz)
+    j, k = (ic // nx) % ny, ic // (nx * ny)
+    inner = i0 & (j > 0) & (j < ny - 1) & (k > 0) & (k < nz - 1)
+    assert np.array_equal(c.arrays["u0"][inner], ref.arrays["u0"][inner])
+    assert (ref.arrays["u0"][inner] > 0).all()
+    xe = float(nx - 1)
+    tris_out = [((xe, lo, lo), (xe, hi, lo), (xe, hi, hi)), ((xe, lo, lo), (xe, hi, hi), (xe, lo, hi))]
+    om0 = c.arrays["omega"].copy()
+    out = geo.find_points(_patch(tris_out, (1.0, 0.0, 0.0)), nx, ny, nz)
+    cases.apply_patch(c, out, ("Outlet", "Cyclic", "Cyclic"), spz=3)
+    changed = (c.arrays["omega"] != om0).reshape(nz, ny, nx)
+    assert changed.any() and not changed[:, :, : nx - 3].any()
+    assert (c.arrays["boundary1"].reshape(nz, ny, nx)[:, :, nx - 1] == lb.BC["Outlet"]).all()
