@@ -65,6 +65,9 @@ struct StepArgs {
     int sponge_mask;  // faces whose centre cell is Outlet or FixedDensity (tauSpongeZone)
     Real inv_spx, inv_spy, inv_spz;  // 1 / (n * spzWidth) per axis
     int tune;         // TUNE_* launch options chosen by the autotuner
+    // y-tiled cell order (TUNE_YTILE): tiles of ty rows, plane by plane inside a tile
+    uint32_t tile_cells, tile_plane, nxny;  // nx*ty*nz, nx*ty, nx*ny
+    FastDiv dtile, dplane;
 };
 
 template <typename Real>
@@ -74,7 +77,22 @@ using Step19Args = StepArgs<Real, 19>;
 enum : int {
     TUNE_NT_STORE = 1,  // accepted, no effect: the populations are always stored non-temporally
     TUNE_NO_XCD = 2,    // keep the hardware work-group order (no XCD slab remap)
+    TUNE_YTILE = 4,     // visit the cells in y-tiles (see ytile_cell)
 };
+
+// Cell of position lin in the y-tiled order: tiles of ty full rows (ny a multiple of ty),
+// each walked plane by plane (k), row by row inside a plane.  The velocity gradient of
+// WALE reads the planes k - 1 and k + 1 of a cell's velocities: over a tile of 32 rows of
+// 1024 cells those three planes of three arrays take 1.2 MB and stay in one XCD's L2
+// between their uses, where whole planes (1.5 MB each) do not.
+template <typename A>
+__device__ inline uint32_t ytile_cell(const A &a, uint32_t lin)
+{
+    const uint32_t tile = fdiv(lin, a.dtile);
+    const uint32_t rem = lin - tile * a.tile_cells;
+    const uint32_t k = fdiv(rem, a.dplane);
+    return (rem - k * a.tile_plane) + tile * a.tile_plane + k * a.nxny;
+}
 
 // XCD-aware block order: the hardware deals work-groups round-robin over the 8 XCDs;
 // renumber them so that each XCD walks one contiguous slab of cells and the partial
@@ -661,8 +679,9 @@ template <typename S, typename Real, int SGS, bool IBB = false, int AA = 0>
 __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
 {
     const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t icu = blk * blockDim.x + threadIdx.x;
-    if (icu >= a.N) return;
+    const uint32_t lin = blk * blockDim.x + threadIdx.x;
+    if (lin >= a.N) return;
+    const uint32_t icu = (a.tune & TUNE_YTILE) ? ytile_cell(a, lin) : lin;
 #define CELL_RETURN return
 #define CELL_IBB IBB
 #define CELL_WALE_INTERIOR true

@@ -536,8 +536,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStep2
 step27_kernel(const StepArgs<Real, 27> a)
 {
     const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t icu = blk * blockDim.x + threadIdx.x;
-    if (icu >= a.N) return;
+    const uint32_t lin = blk * blockDim.x + threadIdx.x;
+    if (lin >= a.N) return;
+    const uint32_t icu = (a.tune & TUNE_YTILE) ? ytile_cell(a, lin) : lin;
 #define CELL_RETURN return
 #define CELL_WALE_INTERIOR true
 #include "d3q27_cell.inc"

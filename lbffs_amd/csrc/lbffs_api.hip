@@ -439,6 +439,15 @@ int pack_flags(lbffs_ctx *c)
     return 0;
 }
 
+// rows per tile of the y-tiled cell order: 32, 16 or 8, the first that divides ny (0: the
+// order is not available on this lattice)
+uint32_t ytile_rows(const lbffs_ctx *c)
+{
+    for (uint32_t t : {32u, 16u, 8u})
+        if ((uint32_t)c->d.ny % t == 0 && (uint32_t)c->d.ny > t) return t;
+    return 0;
+}
+
 template <typename S, typename Real, int Q>
 void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
 {
@@ -487,6 +496,16 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
     a.inv_spy = (Real)(spz > 0 ? 1.0 / ((double)ny * spz) : 0.0);
     a.inv_spz = (Real)(spz > 0 ? 1.0 / ((double)nz * spz) : 0.0);
     a.tune = c->tune;
+    {
+        const uint32_t ty = ytile_rows(c);
+        const uint32_t t = ty ? ty : (uint32_t)ny;
+        a.tile_plane = (uint32_t)nx * t;
+        a.tile_cells = a.tile_plane * (uint32_t)nz;
+        a.nxny = (uint32_t)nx * (uint32_t)ny;
+        a.dtile = make_fastdiv(a.tile_cells);
+        a.dplane = make_fastdiv(a.tile_plane);
+        if (!ty) a.tune &= ~TUNE_YTILE;
+    }
 }
 
 // aa: 0 two-lattice streaming, 1 or 2 the first or second in-place half-step (D3Q19)
@@ -1146,11 +1165,12 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
         he = hipMemcpyAsync((char *)saved_r + k * n * rs, keep[k], n * rs, hipMemcpyDeviceToDevice, c->stream);
     if (saved_m && he == hipSuccess) he = hipMemcpyAsync(saved_m, c->markers, mbytes, hipMemcpyDeviceToDevice, c->stream);
     const int blocks[3] = {64, 128, 256};
-    const int tunes[2] = {0, TUNE_NO_XCD};
+    const int tunes[3] = {0, TUNE_NO_XCD, TUNE_YTILE};
+    const int ntunes = ytile_rows(c) ? 3 : 2;
     float bms = 1e30f;
     int bb = block0, bt = tune0;
     for (int bi = 0; bi < 3 && he == hipSuccess && rc == 0; bi++) {
-        for (int ti = 0; ti < 2 && he == hipSuccess && rc == 0; ti++) {
+        for (int ti = 0; ti < ntunes && he == hipSuccess && rc == 0; ti++) {
             c->block = blocks[bi];
             c->tune = tunes[ti];
             rc = dispatch<StepOp>(c, 1, p);  // warm the shape
@@ -1207,7 +1227,7 @@ int lbffs_set_launch(lbffs_ctx *c, int block, int tune)
 {
     if (!c) return fail(LBFFS_E_INVALID, "null argument");
     if (block % 64 != 0 || block < 64 || block > 256) return fail(LBFFS_E_INVALID, "block must be 64, 128, 192 or 256");
-    if (tune & ~(TUNE_NT_STORE | TUNE_NO_XCD)) return fail(LBFFS_E_INVALID, "unknown launch option");
+    if (tune & ~(TUNE_NT_STORE | TUNE_NO_XCD | TUNE_YTILE)) return fail(LBFFS_E_INVALID, "unknown launch option");
     c->block = block;
     c->tune = tune;
     return 0;

@@ -111,13 +111,16 @@ def test_block_size_invariance(block):
     assert np.array_equal(ref["f"], h["f"])
 
 
-@pytest.mark.parametrize("tune", [1, 2, 3])
-def test_launch_options_bitwise_identical(tune):
-    # non-temporal stores and the work-group order change no arithmetic
-    c = cases.cylinder_channel(nx=64, ny=32, nz=4, D=8, cx=20, cy=16, spz_cells=6, sgs=lb.SGS_SMAGORINSKY)
+@pytest.mark.parametrize("sgs", [lb.SGS_SMAGORINSKY, lb.SGS_WALE])
+@pytest.mark.parametrize("tune", [1, 2, 3, 4, 6])
+def test_launch_options_bitwise_identical(tune, sgs):
+    # non-temporal stores, the work-group order and the y-tiled cell order (4: tiles of 16
+    # rows here) change no arithmetic
+    c = cases.cylinder_channel(nx=64, ny=32, nz=4, D=8, cx=20, cy=16, spz_cells=6, sgs=sgs)
     a = run_hip(c, 25)
     b = run_hip(c, 25, tune=tune)
-    assert np.array_equal(a["f"], b["f"])
+    for k in ("f", "u", "tauSGS", "Fwx"):
+        assert np.array_equal(a[k], b[k]), k
 
 
 def test_autotune_restores_state():
