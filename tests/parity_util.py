@@ -81,7 +81,7 @@ def parity(case, nsteps, keys=("rho", "vel", "f"), tau_tol=1e-4, label="", **kw)
     floating-point contraction (the reference kernels are compiled with contraction),
     each also started one ulp away (the flow's sensitivity to rounding is part of the
     level).  Pass if
-      |hip32 - oracle32| <= max(1e-5, 2 floor)      (the north-star 1e-5 where FP32 allows it)
+      |hip32 - oracle32| <= max(1e-5, 2 floor)      (the 1e-5 parity bound where FP32 allows it)
       |hip32 - oracle64| <= 2 floor + 1e-6          (no less accurate than a literal FP32 run)
     tauSGS (a derived, strongly nonlinear field) is held to tau_tol against oracle32.
     """

@@ -98,3 +98,17 @@ def test_streaming_descriptor_checks():
     descs = (lb.Desc * 1)(lb.Desc(abi_version=lb.ABI_VERSION, q=19, nx=8, ny=8, nz=8, omega0=1.0,
                                   streaming=lb.STREAM_AA))
     assert L.lbffs_batch_create(descs, 1, ctypes.byref(h)) == -4
+
+
+def test_geometry_and_patch_argument_checks():
+    # the context q-distance cast and findPoints refuse bad arguments before any device call
+    L = lb.lib()
+    miss = ctypes.c_int()
+    assert L.lbffs_geom_qlinks_ctx(None, 0, None, None, None, ctypes.c_float(0.5), ctypes.byref(miss)) == -1
+    cnt = ctypes.c_long()
+    assert L.lbffs_case_find_points(-1, None, None, None, None, None, 4, 4, 4, None, 0, ctypes.byref(cnt)) != 0
+    assert L.lbffs_case_find_points(1, None, None, None, None, None, 4, 4, 4, None, 0, ctypes.byref(cnt)) != 0
+    assert L.lbffs_case_find_points(0, None, None, None, None, None, 4, 4, 4, None, 0, ctypes.byref(cnt)) == 0
+    assert cnt.value == 0
+    # launch options: a null context is refused before the options are read
+    assert L.lbffs_set_launch(None, 256, 8) == -1
