@@ -3,6 +3,7 @@ include/*.h declares, and reports errors as codes (no compute calls here)."""
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -112,3 +113,30 @@ def test_geometry_and_patch_argument_checks():
     assert cnt.value == 0
     # launch options: a null context is refused before the options are read
     assert L.lbffs_set_launch(None, 256, 8) == -1
+
+
+def test_headers_compile_as_c_and_link(tmp_path):
+    # the drop-in boundary is plain C: every header compiles as C99 (gcc), and a C host that
+    # calls one entry point of each header links against the library without HIP headers
+    inc = os.path.join(ROOT, "include")
+    src = tmp_path / "host.c"
+    src.write_text(
+        '#include "lbffs.h"\n#include "lbffs_case.h"\n#include "lbffs_geom.h"\n#include "lbffs_output.h"\n'
+        "#include <stdio.h>\n"
+        "int main(void) {\n"
+        "  lbffs_ctx *c = 0; lbffs_desc d = {0}; long n = 0;\n"
+        "  int e = lbffs_create(0, &c);\n"
+        "  int f = lbffs_case_find_points(0, 0, 0, 0, 0, 0, 4, 4, 4, 0, 0, &n);\n"
+        "  int g = lbffs_geom_qlinks_ctx(0, 0, 0, 0, 0, 0.5f, 0);\n"
+        "  int h = lbffs_out_write_vtk(0, 0, LBFFS_VTK_ASCII);\n"
+        "  (void)d; printf(\"%d %d %d %d %s\\n\", e, f, g, h, lbffs_strerror(e));\n"
+        "  return 0;\n}\n")
+    exe = tmp_path / "host"
+    lib_dir = os.path.join(ROOT, "lbffs_amd")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", inc, str(src), "-L", lib_dir, "-llbffs",
+                        "-Wl,-rpath," + lib_dir, "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    e, f, g, h = (int(x) for x in out.stdout.split()[:4])
+    assert e == -1 and f == 0 and g == -1 and h != 0
