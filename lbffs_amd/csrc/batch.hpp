@@ -14,7 +14,9 @@ namespace lbffs {
 
 struct BatchBlock {
     uint32_t lattice;  // index into the argument array
-    uint32_t first;    // first cell of this 256-cell block in its lattice
+    uint32_t first;    // first cell of this 256-cell block in its lattice, or (face table)
+                       // first entry of the block in the face-cell list
+    uint32_t count;    // face table: cells of the block (the others: unused)
 };
 
 constexpr int kBatchBlock = 256;
@@ -23,27 +25,40 @@ constexpr int kBatchBlock = 256;
 // (uniform per work-group, so through scalar loads); the host prepares a separate argument
 // array for the last step of a call, whose cells store every moment and diagnostic.
 // IBB: D3Q19's interpolated bounce-back compiled in (chosen when any lattice of the batch
-// asks for it); the lattices without it keep simple bounce-back through their own ibb flag
-template <typename S, typename Real, int SGS, int Q, bool IBB>
+// asks for it); the lattices without it keep simple bounce-back through their own ibb flag.
+// PASS 1 runs the cells of the block table that are not on a lattice face; PASS 2 runs the
+// face cells of the face table (cell indices from `cells`).  On small lattices nearly every
+// wave holds a face cell (i = 0 or nx - 1), and a wave with both kinds runs the face path
+// and the interior pull: split in two launches, each wave runs one of them.  Both passes
+// read only the previous step's arrays, so their order does not matter.
+template <typename S, typename Real, int SGS, int Q, bool IBB, int PASS>
 __global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Q == 27 ? kBatch27MinWaves<Real> : 1, 8)))
     batch_step_kernel(const StepArgs<Real, Q> *__restrict__ args, const BatchBlock *__restrict__ table,
-                      uint32_t nblocks)
+                      uint32_t nblocks, const uint32_t *__restrict__ cells)
 {
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         const BatchBlock e = table[b];
         const StepArgs<Real, Q> &a = args[e.lattice];
-        const uint32_t icu = e.first + threadIdx.x;
-        if (icu >= a.N) continue;
+        uint32_t icu;
+        if constexpr (PASS == 2) {
+            if (threadIdx.x >= e.count) continue;
+            icu = cells[e.first + threadIdx.x];
+        } else {
+            icu = e.first + threadIdx.x;
+            if (icu >= a.N) continue;
+        }
         do {
 #define CELL_RETURN continue
 #define CELL_IBB (IBB && a.ibb != 0)
 #define CELL_WALE_INTERIOR true
 #define CELL_AA 0
+#define CELL_SKIP_FACE (PASS == 1)
             if constexpr (Q == 27) {
 #include "d3q27_cell.inc"
             } else {
 #include "d3q19_cell.inc"
             }
+#undef CELL_SKIP_FACE
 #undef CELL_AA
 #undef CELL_WALE_INTERIOR
 #undef CELL_IBB

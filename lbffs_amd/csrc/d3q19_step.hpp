@@ -686,7 +686,9 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
 #define CELL_IBB IBB
 #define CELL_WALE_INTERIOR true
 #define CELL_AA AA
+#define CELL_SKIP_FACE false
 #include "d3q19_cell.inc"
+#undef CELL_SKIP_FACE
 #undef CELL_AA
 #undef CELL_WALE_INTERIOR
 #undef CELL_IBB

@@ -541,7 +541,9 @@ step27_kernel(const StepArgs<Real, 27> a)
     const uint32_t icu = (a.tune & TUNE_YTILE) ? ytile_cell(a, lin) : lin;
 #define CELL_RETURN return
 #define CELL_WALE_INTERIOR true
+#define CELL_SKIP_FACE false
 #include "d3q27_cell.inc"
+#undef CELL_SKIP_FACE
 #undef CELL_WALE_INTERIOR
 #undef CELL_RETURN
 }

@@ -835,6 +835,11 @@ struct lbffs_batch {
     std::vector<char> args_host[4];
     BatchBlock *table = nullptr;
     uint32_t nblocks = 0;
+    // the face cells of every lattice (run by a launch of their own), rebuilt when flags change
+    BatchBlock *ftable = nullptr;
+    uint32_t nfblocks = 0;
+    uint32_t *fcells = nullptr;
+    bool ftable_ok = false;
     int grid = 0;        // work-groups of the launch; 0 = one per table entry
     int graph_chunk = 0; // steps per captured hipGraph; 0 = plain launches
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -858,10 +863,49 @@ void batch_drop_graphs(lbffs_batch *b)
 template <typename S, typename Real, int Q, int SGS, bool IBB>
 void batch_launch(lbffs_batch *b, int parity, int last)
 {
+    const StepArgs<Real, Q> *args = (const StepArgs<Real, Q> *)b->args_dev[parity + 2 * last];
     const unsigned grid = b->grid > 0 ? (unsigned)b->grid : b->nblocks;
-    hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, IBB>), dim3(grid), dim3(kBatchBlock), 0, b->stream,
-                       (const StepArgs<Real, Q> *)b->args_dev[parity + 2 * last], (const BatchBlock *)b->table,
-                       b->nblocks);
+    hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, IBB, 1>), dim3(grid), dim3(kBatchBlock), 0, b->stream, args,
+                       (const BatchBlock *)b->table, b->nblocks, (const uint32_t *)nullptr);
+    if (b->nfblocks) {
+        const unsigned fgrid = b->grid > 0 ? std::min((unsigned)b->grid, b->nfblocks) : b->nfblocks;
+        hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, IBB, 2>), dim3(fgrid), dim3(kBatchBlock), 0, b->stream,
+                           args, (const BatchBlock *)b->ftable, b->nfblocks, (const uint32_t *)b->fcells);
+    }
+}
+
+// the face table: every non-solid face cell of every lattice, 256 to a block
+int batch_face_table(lbffs_batch *b)
+{
+    std::vector<uint32_t> cells;
+    std::vector<BatchBlock> blocks;
+    std::vector<uint16_t> fl;
+    for (size_t d = 0; d < b->ctx.size(); d++) {
+        lbffs_ctx *c = b->ctx[d];
+        fl.resize((size_t)c->N);
+        HIP_TRY(hipMemcpy(fl.data(), c->flag, (size_t)c->N * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        const size_t start = cells.size();
+        for (long ic = 0; ic < c->N; ic++)
+            if ((fl[ic] & F_FACE) && !(fl[ic] & F_SOLID)) cells.push_back((uint32_t)ic);
+        for (size_t f = start; f < cells.size(); f += kBatchBlock)
+            blocks.push_back({(uint32_t)d, (uint32_t)f, (uint32_t)std::min<size_t>(kBatchBlock, cells.size() - f)});
+    }
+    if (cells.size() > 0xffffffffu) return fail(LBFFS_E_INVALID, "too many face cells in a batch");
+    if (b->ftable) (void)hipFree(b->ftable);
+    if (b->fcells) (void)hipFree(b->fcells);
+    b->ftable = nullptr;
+    b->fcells = nullptr;
+    b->nfblocks = 0;
+    if (!blocks.empty()) {
+        if (int e = dalloc((void **)&b->ftable, blocks.size() * sizeof(BatchBlock))) return e;
+        if (int e = dalloc((void **)&b->fcells, cells.size() * sizeof(uint32_t))) return e;
+        HIP_TRY(hipMemcpy(b->ftable, blocks.data(), blocks.size() * sizeof(BatchBlock), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->fcells, cells.data(), cells.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        b->nfblocks = (uint32_t)blocks.size();
+    }
+    batch_drop_graphs(b);  // the captured launches name the old table
+    b->ftable_ok = true;
+    return 0;
 }
 
 template <typename S, typename Real, int Q, int SGS, bool IBB>
@@ -974,7 +1018,8 @@ void batch_free(lbffs_batch *b)
         free_all(c);
         delete c;
     }
-    for (void *p : {b->args_dev[0], b->args_dev[1], b->args_dev[2], b->args_dev[3], (void *)b->table})
+    for (void *p : {b->args_dev[0], b->args_dev[1], b->args_dev[2], b->args_dev[3], (void *)b->table,
+                    (void *)b->ftable, (void *)b->fcells})
         if (p) (void)hipFree(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1400,7 +1445,7 @@ int lbffs_batch_create(const lbffs_desc *descs, int n, lbffs_batch **out)
         c->stream = b->stream;
         c->owns_stream = false;
         b->ctx.push_back(c);
-        for (long first = 0; first < c->N; first += kBatchBlock) table.push_back({(uint32_t)d, (uint32_t)first});
+        for (long first = 0; first < c->N; first += kBatchBlock) table.push_back({(uint32_t)d, (uint32_t)first, 0u});
     }
     b->nblocks = (uint32_t)table.size();
     const size_t amax = sizeof(StepArgs<double, 27>) * (size_t)n;
@@ -1453,7 +1498,11 @@ int lbffs_batch_step(lbffs_batch *b, int nsteps, const lbffs_step_params *params
         if (c->flags_dirty) {
             int e = is_f64(c) ? pack_flags<double>(c) : pack_flags<float>(c);
             if (e) return fail(e, "lattice " + std::to_string(d) + ": " + g_last_error);
+            b->ftable_ok = false;
         }
+    }
+    if (!b->ftable_ok) {
+        if (int e = batch_face_table(b)) return e;
     }
     if (nsteps == 0) return 0;
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
