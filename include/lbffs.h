@@ -184,9 +184,12 @@ int lbffs_output_macros(lbffs_ctx *ctx, float dpdx, void *rho, void *u, void *v,
 enum {
     LBFFS_TUNE_NT_STORE = 1, /* accepted, no effect: new populations are always stored non-temporally */
     LBFFS_TUNE_NO_XCD = 2,   /* keep the hardware work-group order across the 8 XCDs */
-    LBFFS_TUNE_YTILE = 4     /* visit the cells in tiles of 32, 16 or 8 rows (the first that
+    LBFFS_TUNE_YTILE = 4,    /* visit the cells in tiles of 32, 16 or 8 rows (the first that
                                 divides ny), plane by plane: the neighbour planes WALE reads
                                 stay in L2; ignored where no such tile size exists */
+    LBFFS_TUNE_SPLIT_FACES = 8 /* two launches per step: the cells off the lattice faces, then
+                                  the face cells, so that no wave runs both paths (two-lattice
+                                  streaming; ignored by in-place half-steps) */
 };
 
 /* Autotuner: time every candidate work-group size (64, 128, 256) and launch option on

@@ -68,6 +68,8 @@ struct StepArgs {
     // y-tiled cell order (TUNE_YTILE): tiles of ty rows, plane by plane inside a tile
     uint32_t tile_cells, tile_plane, nxny;  // nx*ty*nz, nx*ty, nx*ny
     FastDiv dtile, dplane;
+    const uint32_t *fcells;  // the lattice's non-solid face cells (the face pass runs these)
+    uint32_t nface;
 };
 
 template <typename Real>
@@ -78,6 +80,7 @@ enum : int {
     TUNE_NT_STORE = 1,  // accepted, no effect: the populations are always stored non-temporally
     TUNE_NO_XCD = 2,    // keep the hardware work-group order (no XCD slab remap)
     TUNE_YTILE = 4,     // visit the cells in y-tiles (see ytile_cell)
+    TUNE_SPLIT = 8,     // two passes per step: the cells off the faces, then the face cells
 };
 
 // Cell of position lin in the y-tiled order: tiles of ty full rows (ny a multiple of ty),
@@ -674,19 +677,30 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
 // IBB selects the interpolated bounce-back at compile time: its loop raises the register
 // count of the whole kernel from 63 to 72 VGPRs (7 instead of 8 waves per SIMD), which the
 // 16-bit forms, bound by instruction issue, feel.  AA: 0 two-lattice streaming, 1 and 2
-// the two half-steps of in-place streaming (the cell body describes them).
-template <typename S, typename Real, int SGS, bool IBB = false, int AA = 0>
+// the two half-steps of in-place streaming (the cell body describes them).  PASS: 0 every
+// cell; 1 every cell but the lattice-face cells; 2 the face cells of the list a.fcells.  A
+// wave that holds face and interior cells runs both the face path and the interior pull;
+// two passes keep them apart (TUNE_SPLIT runs 1 then 2; both read only the previous
+// step's arrays).  It pays where such waves are many and the step is bound by instruction
+// issue, and costs a second launch's drain where the step is bound by HBM.
+template <typename S, typename Real, int SGS, bool IBB = false, int AA = 0, int PASS = 0>
 __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
 {
     const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     const uint32_t lin = blk * blockDim.x + threadIdx.x;
-    if (lin >= a.N) return;
-    const uint32_t icu = (a.tune & TUNE_YTILE) ? ytile_cell(a, lin) : lin;
+    uint32_t icu;
+    if constexpr (PASS == 2) {
+        if (lin >= a.nface) return;
+        icu = a.fcells[lin];
+    } else {
+        if (lin >= a.N) return;
+        icu = (a.tune & TUNE_YTILE) ? ytile_cell(a, lin) : lin;
+    }
 #define CELL_RETURN return
 #define CELL_IBB IBB
 #define CELL_WALE_INTERIOR true
 #define CELL_AA AA
-#define CELL_SKIP_FACE false
+#define CELL_SKIP_FACE (PASS == 1)
 #include "d3q19_cell.inc"
 #undef CELL_SKIP_FACE
 #undef CELL_AA

@@ -531,17 +531,24 @@ constexpr int kStep27MinWaves = sizeof(Real) == 8 ? 1 : 7;
 template <typename Real>
 constexpr int kBatch27MinWaves = sizeof(Real) == 8 ? 1 : 6;
 
-template <typename S, typename Real, int SGS>
+// PASS as step19_kernel: 1 the cells off the lattice faces, 2 the listed face cells
+template <typename S, typename Real, int SGS, int PASS = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStep27MinWaves<Real>, 8)))
 step27_kernel(const StepArgs<Real, 27> a)
 {
     const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
     const uint32_t lin = blk * blockDim.x + threadIdx.x;
-    if (lin >= a.N) return;
-    const uint32_t icu = (a.tune & TUNE_YTILE) ? ytile_cell(a, lin) : lin;
+    uint32_t icu;
+    if constexpr (PASS == 2) {
+        if (lin >= a.nface) return;
+        icu = a.fcells[lin];
+    } else {
+        if (lin >= a.N) return;
+        icu = (a.tune & TUNE_YTILE) ? ytile_cell(a, lin) : lin;
+    }
 #define CELL_RETURN return
 #define CELL_WALE_INTERIOR true
-#define CELL_SKIP_FACE false
+#define CELL_SKIP_FACE (PASS == 1)
 #include "d3q27_cell.inc"
 #undef CELL_SKIP_FACE
 #undef CELL_WALE_INTERIOR

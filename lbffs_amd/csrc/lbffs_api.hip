@@ -64,6 +64,8 @@ struct lbffs_ctx {
     void *f[2] = {nullptr, nullptr};  // one array (f[0]) with in-place streaming
     int cur = 0;
     int aa_odd = 0;  // in-place streaming: the array is in the layout of a first half-step
+    uint32_t *fcells = nullptr;  // non-solid face cells (the face pass), rebuilt with the flags
+    uint32_t nface = 0;
     int32_t *boundary = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
     uint8_t *solid = nullptr, *neiSolid = nullptr;
     void *sdf = nullptr, *u0 = nullptr, *v0 = nullptr, *w0 = nullptr, *omega = nullptr;
@@ -435,6 +437,22 @@ int pack_flags(lbffs_ctx *c)
     c->sponge_mask = 0;
     for (int s = 0; s < 6; s++)
         if (codes[s] == BC_OUTLET || codes[s] == BC_FIXEDDENSITY) c->sponge_mask |= 1 << s;
+    // the face pass's list: the non-solid lattice-face cells, in index order
+    {
+        std::vector<uint16_t> fl((size_t)c->N);
+        HIP_TRY(hipMemcpy(fl.data(), c->flag, (size_t)c->N * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        std::vector<uint32_t> cells;
+        for (long ic = 0; ic < c->N; ic++)
+            if ((fl[ic] & F_FACE) && !(fl[ic] & F_SOLID)) cells.push_back((uint32_t)ic);
+        if (c->fcells) (void)hipFree(c->fcells);
+        c->fcells = nullptr;
+        c->nface = 0;
+        if (!cells.empty()) {
+            if (int e = dalloc((void **)&c->fcells, cells.size() * sizeof(uint32_t))) return e;
+            HIP_TRY(hipMemcpy(c->fcells, cells.data(), cells.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            c->nface = (uint32_t)cells.size();
+        }
+    }
     c->flags_dirty = false;
     return 0;
 }
@@ -496,6 +514,8 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
     a.inv_spy = (Real)(spz > 0 ? 1.0 / ((double)ny * spz) : 0.0);
     a.inv_spz = (Real)(spz > 0 ? 1.0 / ((double)nz * spz) : 0.0);
     a.tune = c->tune;
+    a.fcells = c->fcells;
+    a.nface = c->nface;
     {
         const uint32_t ty = ytile_rows(c);
         const uint32_t t = ty ? ty : (uint32_t)ny;
@@ -508,21 +528,39 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
     }
 }
 
-// aa: 0 two-lattice streaming, 1 or 2 the first or second in-place half-step (D3Q19)
+// aa: 0 two-lattice streaming, 1 or 2 the first or second in-place half-step (D3Q19).
+// With TUNE_SPLIT a two-lattice step runs in two passes: the cells off the lattice faces,
+// then the face cells of the list (a.fcells); otherwise, and for in-place half-steps, one.
 template <typename S, typename Real, int Q, int SGS>
 void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream_t st, int aa)
 {
+    const bool split = (a.tune & TUNE_SPLIT) && aa == 0;
+    const unsigned fgrid = (a.nface + (unsigned)block - 1) / (unsigned)block;
     if constexpr (Q == 27) {
-        hipLaunchKernelGGL((step27_kernel<S, Real, SGS>), dim3(grid), dim3(block), 0, st, a);
+        if (split) {
+            hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 1>), dim3(grid), dim3(block), 0, st, a);
+            if (fgrid) hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 2>), dim3(fgrid), dim3(block), 0, st, a);
+        } else {
+            hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 0>), dim3(grid), dim3(block), 0, st, a);
+        }
     } else {
         if (aa == 1)
             hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 1>), dim3(grid), dim3(block), 0, st, a);
         else if (aa == 2)
             hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 2>), dim3(grid), dim3(block), 0, st, a);
-        else if (a.ibb)
-            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true>), dim3(grid), dim3(block), 0, st, a);
-        else
-            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false>), dim3(grid), dim3(block), 0, st, a);
+        else if (a.ibb && split) {
+            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true, 0, 1>), dim3(grid), dim3(block), 0, st, a);
+            if (fgrid)
+                hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true, 0, 2>), dim3(fgrid), dim3(block), 0, st, a);
+        } else if (a.ibb) {
+            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true, 0, 0>), dim3(grid), dim3(block), 0, st, a);
+        } else if (split) {
+            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 0, 1>), dim3(grid), dim3(block), 0, st, a);
+            if (fgrid)
+                hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 0, 2>), dim3(fgrid), dim3(block), 0, st, a);
+        } else {
+            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 0, 0>), dim3(grid), dim3(block), 0, st, a);
+        }
     }
 }
 
@@ -817,7 +855,7 @@ void free_all(lbffs_ctx *c)
     void *ptrs[] = {c->f[0], c->f[1], c->boundary, c->b1, c->b2, c->b3, c->solid, c->neiSolid, c->sdf, c->u0, c->v0,
                     c->w0, c->omega, c->Fwx, c->Fwy, c->Fwz, c->tau, c->Gx, c->Gy, c->Gz, c->rho[0], c->rho[1],
                     c->u[0], c->u[1], c->v[0], c->v[1], c->w[0], c->w[1], c->flag, c->d_err, c->d_part, c->d_red,
-                    c->markers, c->Gm, c->mark, c->list, c->count, c->qlink};
+                    c->markers, c->Gm, c->mark, c->list, c->count, c->qlink, c->fcells};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -1210,8 +1248,12 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
         he = hipMemcpyAsync((char *)saved_r + k * n * rs, keep[k], n * rs, hipMemcpyDeviceToDevice, c->stream);
     if (saved_m && he == hipSuccess) he = hipMemcpyAsync(saved_m, c->markers, mbytes, hipMemcpyDeviceToDevice, c->stream);
     const int blocks[3] = {64, 128, 256};
-    const int tunes[3] = {0, TUNE_NO_XCD, TUNE_YTILE};
-    const int ntunes = ytile_rows(c) ? 3 : 2;
+    int tunes[6] = {0, TUNE_NO_XCD, TUNE_SPLIT, TUNE_SPLIT | TUNE_NO_XCD, TUNE_YTILE, TUNE_SPLIT | TUNE_YTILE};
+    int ntunes = ytile_rows(c) ? 6 : 4;
+    if (c->d.streaming == LBFFS_STREAM_AA) {  // one pass per half-step
+        tunes[2] = TUNE_YTILE;
+        ntunes = ytile_rows(c) ? 3 : 2;
+    }
     float bms = 1e30f;
     int bb = block0, bt = tune0;
     for (int bi = 0; bi < 3 && he == hipSuccess && rc == 0; bi++) {
@@ -1272,7 +1314,8 @@ int lbffs_set_launch(lbffs_ctx *c, int block, int tune)
 {
     if (!c) return fail(LBFFS_E_INVALID, "null argument");
     if (block % 64 != 0 || block < 64 || block > 256) return fail(LBFFS_E_INVALID, "block must be 64, 128, 192 or 256");
-    if (tune & ~(TUNE_NT_STORE | TUNE_NO_XCD | TUNE_YTILE)) return fail(LBFFS_E_INVALID, "unknown launch option");
+    if (tune & ~(TUNE_NT_STORE | TUNE_NO_XCD | TUNE_YTILE | TUNE_SPLIT))
+        return fail(LBFFS_E_INVALID, "unknown launch option");
     c->block = block;
     c->tune = tune;
     return 0;

@@ -112,10 +112,10 @@ def test_block_size_invariance(block):
 
 
 @pytest.mark.parametrize("sgs", [lb.SGS_SMAGORINSKY, lb.SGS_WALE])
-@pytest.mark.parametrize("tune", [1, 2, 3, 4, 6])
+@pytest.mark.parametrize("tune", [1, 2, 3, 4, 6, 8, 10, 12])
 def test_launch_options_bitwise_identical(tune, sgs):
-    # non-temporal stores, the work-group order and the y-tiled cell order (4: tiles of 16
-    # rows here) change no arithmetic
+    # non-temporal stores, the work-group order, the y-tiled cell order (4: tiles of 16
+    # rows here) and the split face pass (8) change no arithmetic
     c = cases.cylinder_channel(nx=64, ny=32, nz=4, D=8, cx=20, cy=16, spz_cells=6, sgs=sgs)
     a = run_hip(c, 25)
     b = run_hip(c, 25, tune=tune)
