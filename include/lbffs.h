@@ -187,9 +187,16 @@ enum {
     LBFFS_TUNE_YTILE = 4,    /* visit the cells in tiles of 32, 16 or 8 rows (the first that
                                 divides ny), plane by plane: the neighbour planes WALE reads
                                 stay in L2; ignored where no such tile size exists */
-    LBFFS_TUNE_SPLIT_FACES = 8 /* two launches per step: the cells off the lattice faces, then
-                                  the face cells, so that no wave runs both paths (two-lattice
-                                  streaming; ignored by in-place half-steps) */
+    LBFFS_TUNE_SPLIT_FACES = 8, /* two launches per step: the cells off the lattice faces, then
+                                   the face cells, so that no wave runs both paths (two-lattice
+                                   streaming; ignored by in-place half-steps) */
+    LBFFS_TUNE_NO_PURE = 16,    /* every wave reads its flags before it pulls; by default a wave
+                                   of 64 interior fluid cells (a per-wave summary of the flags
+                                   says so) issues its pulls together with the flag load */
+    LBFFS_TUNE_FACE_FIRST = 32  /* one launch per step whose first work-groups run the face cells
+                                   (from a list) and the others the cells off the faces, so that
+                                   no wave runs both paths (two-lattice streaming; takes
+                                   precedence over LBFFS_TUNE_SPLIT_FACES) */
 };
 
 /* Autotuner: time every candidate work-group size (64, 128, 256) and launch option on
@@ -268,6 +275,23 @@ const char *lbffs_last_error(void);
  * stores; 2: read only; 3: write only.  lbffs_probe_copy is mode 0. */
 int lbffs_probe_bandwidth(int mode, size_t bytes, int reps, float *ms_per_pass);
 int lbffs_probe_copy(size_t bytes, int reps, float *ms_per_copy);
+
+/* Diagnostics, no counterpart in the reference: a configurable streaming probe over
+ * `bytes` (multiple of 16).  kind 0 copy, 1 read only, 2 write only; unroll = 1, 2, 4, 8
+ * or 16 float4 per work-item, all loads issued before the stores; nt = 1: non-temporal
+ * loads and stores; block = work-group size (multiple of 64, at most 1024); grid = work-
+ * groups (0: one per tile of unroll*block float4, otherwise a grid-stride walk over tiles). */
+int lbffs_probe_stream(int kind, int unroll, int nt, int block, int grid, size_t bytes, int reps, float *ms_per_pass);
+
+/* Diagnostics, no counterpart in the reference: the memory skeleton of one lattice step --
+ * every cell of an nx*ny*nz lattice (a multiple of 256 cells) pulls its q (19 or 27)
+ * populations of store_bytes (4 or 2) with the step's offsets and stores them to its own
+ * slots of a second array, with no collision.  variant 0: the lattice offsets (x-shifted
+ * runs straddle cache lines); 1: the x component of every offset dropped (aligned runs:
+ * the ideal pull); 2: x-shifted directions loaded aligned and moved one lane by a DPP wave
+ * shift, the lane leaving the wave loading its value alone; 3: as 2, with that lane's value
+ * passed between the four waves of a work-group through LDS. */
+int lbffs_probe_lattice(int q, int store_bytes, int variant, int nx, int ny, int nz, int reps, float *ms_per_pass);
 
 #ifdef __cplusplus
 }

@@ -105,6 +105,8 @@ def lib():
         L.lbffs_case_find_points.argtypes = [ip, vp, vp, vp, vp, vp, ip, ip, ip, vp, C.c_long, C.POINTER(C.c_long)]
         L.lbffs_probe_copy.argtypes = [sz, ip, C.POINTER(C.c_float)]
         L.lbffs_probe_bandwidth.argtypes = [ip, sz, ip, C.POINTER(C.c_float)]
+        L.lbffs_probe_stream.argtypes = [ip, ip, ip, ip, ip, sz, ip, C.POINTER(C.c_float)]
+        L.lbffs_probe_lattice.argtypes = [ip] * 7 + [C.POINTER(C.c_float)]
         L.lbffs_autotune.argtypes = [vp, C.POINTER(StepParams), ip, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_float)]
         L.lbffs_set_launch.argtypes = [vp, ip, ip]
@@ -134,7 +136,8 @@ def lib():
                   "lbffs_synchronize", "lbffs_last_step_ms", "lbffs_destroy", "lbffs_case_units",
                   "lbffs_case_faces", "lbffs_case_init_feq", "lbffs_case_neisolid", "lbffs_case_outlet_sponge",
                   "lbffs_case_find_points",
-                  "lbffs_probe_copy", "lbffs_probe_bandwidth", "lbffs_autotune", "lbffs_set_launch",
+                  "lbffs_probe_copy", "lbffs_probe_bandwidth", "lbffs_probe_stream", "lbffs_probe_lattice",
+                  "lbffs_autotune", "lbffs_set_launch",
                   "lbffs_ibm_set_markers", "lbffs_ibm_set_motion", "lbffs_ibm_download", "lbffs_batch_create",
                   "lbffs_batch_context", "lbffs_batch_set_launch", "lbffs_batch_step", "lbffs_batch_last_step_ms",
                   "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf", "lbffs_geom_qlinks", "lbffs_geom_qlinks_ctx",
@@ -172,6 +175,23 @@ def probe_bandwidth(mode, nbytes, reps=10):
     """Milliseconds of one pass of probe `mode` over nbytes (lbffs_probe_bandwidth)."""
     ms = C.c_float()
     _check(lib().lbffs_probe_bandwidth(int(mode), int(nbytes), int(reps), C.byref(ms)), "lbffs_probe_bandwidth")
+    return ms.value
+
+
+def probe_stream(kind, unroll, nt, block, grid, nbytes, reps=10):
+    """Milliseconds of one pass of the configurable streaming probe (lbffs_probe_stream):
+    kind 0 copy, 1 read, 2 write."""
+    ms = C.c_float()
+    _check(lib().lbffs_probe_stream(int(kind), int(unroll), int(nt), int(block), int(grid), int(nbytes), int(reps),
+                                    C.byref(ms)), "lbffs_probe_stream")
+    return ms.value
+
+
+def probe_lattice(q, store_bytes, variant, nx, ny, nz, reps=10):
+    """Milliseconds of one pass of the lattice-step memory skeleton (lbffs_probe_lattice)."""
+    ms = C.c_float()
+    _check(lib().lbffs_probe_lattice(int(q), int(store_bytes), int(variant), int(nx), int(ny), int(nz), int(reps),
+                                     C.byref(ms)), "lbffs_probe_lattice")
     return ms.value
 
 

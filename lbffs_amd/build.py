@@ -47,20 +47,37 @@ def _run(cmd, verbose):
         print(out)
 
 
-def build(verbose: bool = True) -> str:
+# kernels.hip is compiled once per (lattice, storage form, single/batch): the step kernels'
+# instantiations build in parallel
+KERNEL_UNITS = [(q, st, batch) for q in (19, 27) for st in range(4) for batch in (0, 1)]
+
+
+def build(verbose: bool = True, jobs: int = 0) -> str:
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(BUILD, exist_ok=True)
+    steps = []
     host_objs = []
     for src in ("lbffs_case.cpp", "lbffs_output.cpp"):
         o = os.path.join(BUILD, src.replace(".cpp", ".o"))
-        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
-              "-c", os.path.join(CSRC, src), "-o", o], verbose)
+        steps.append(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
+                      "-c", os.path.join(CSRC, src), "-o", o])
         host_objs.append(o)
-    tmp = LIB + ".tmp"
     objs = []
+    for q, st, batch in KERNEL_UNITS:
+        o = os.path.join(BUILD, f"kernels_q{q}_s{st}_b{batch}.o")
+        steps.append([HIPCC, *HIP_FLAGS, f"-DKERN_Q={q}", f"-DKERN_STORE={st}", f"-DKERN_BATCH={batch}",
+                      "-c", os.path.join(CSRC, "kernels.hip"), "-o", o])
+        objs.append(o)
     for src in ("lbffs_api.hip", "lbffs_probe.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o], verbose)
+        steps.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
+    n = jobs or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=n) as ex:
+        for f in [ex.submit(_run, cmd, verbose) for cmd in steps]:
+            f.result()
+    tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, *host_objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB

@@ -14,7 +14,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import BC, CaseIn, Units, lib, _check, _ptr, SGS_WALE, SGS_SMAGORINSKY, SGS_NONE
+from . import BC, CaseIn, Units, lib, _check, _ptr, SGS_WALE, SGS_SMAGORINSKY, SGS_NONE, COLL_RR, COLL_CUMULANT
 
 FACES = ["nxMin", "nxMax", "nyMin", "nyMax", "nzMin", "nzMax"]
 
@@ -35,6 +35,14 @@ class Case:
     meta: dict = field(default_factory=dict)
     q: int = 19
     surface: object = None  # an STL whose q-distances are cast on the device after the upload
+    collision: object = None  # LBFFS_COLL_*; None = the reference's (RR on D3Q19, cumulant on D3Q27)
+
+    @property
+    def coll(self):
+        """The collision operator the case runs (an LBFFS_COLL_* value)."""
+        if self.collision is not None:
+            return int(self.collision)
+        return COLL_CUMULANT if self.q == 27 else COLL_RR
 
     @property
     def N(self):

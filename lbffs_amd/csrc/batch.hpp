@@ -53,11 +53,13 @@ __global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_e
 #define CELL_WALE_INTERIOR true
 #define CELL_AA 0
 #define CELL_SKIP_FACE (PASS == 1)
+#define CELL_PURE_OK (PASS == 1)
             if constexpr (Q == 27) {
 #include "d3q27_cell.inc"
             } else {
 #include "d3q19_cell.inc"
             }
+#undef CELL_PURE_OK
 #undef CELL_SKIP_FACE
 #undef CELL_AA
 #undef CELL_WALE_INTERIOR

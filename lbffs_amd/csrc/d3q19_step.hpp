@@ -70,6 +70,9 @@ struct StepArgs {
     FastDiv dtile, dplane;
     const uint32_t *fcells;  // the lattice's non-solid face cells (the face pass runs these)
     uint32_t nface;
+    const uint32_t *wpure;   // per-wave summary (wave_pure), or null: every wave reads its flags first
+    uint32_t step;            // step number (stochastic rounding), see sr_step
+    const uint32_t *step_ctr; // batches: device counter of the steps run in the current call
 };
 
 template <typename Real>
@@ -81,6 +84,8 @@ enum : int {
     TUNE_NO_XCD = 2,    // keep the hardware work-group order (no XCD slab remap)
     TUNE_YTILE = 4,     // visit the cells in y-tiles (see ytile_cell)
     TUNE_SPLIT = 8,     // two passes per step: the cells off the faces, then the face cells
+    TUNE_NO_PURE = 16,  // every wave waits for its flags before it pulls (no per-wave summary)
+    TUNE_FACE_FIRST = 32,  // one launch: the face-list work-groups first, then the cells off the faces
 };
 
 // Cell of position lin in the y-tiled order: tiles of ty full rows (ny a multiple of ty),
@@ -683,16 +688,30 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
 // two passes keep them apart (TUNE_SPLIT runs 1 then 2; both read only the previous
 // step's arrays).  It pays where such waves are many and the step is bound by instruction
 // issue, and costs a second launch's drain where the step is bound by HBM.
+// PASS 3 runs both passes in one grid: the first face_blocks(a) work-groups take the face
+// cells of the list, the others the cells off the faces, so that no wave runs both paths
+// and no second launch waits for the first to drain (the face work-groups are dispatched
+// first and run beside the interior ones).
+template <typename A>
+__device__ inline uint32_t face_blocks(const A &a)
+{
+    return (a.nface + blockDim.x - 1) / blockDim.x;
+}
+
 template <typename S, typename Real, int SGS, bool IBB = false, int AA = 0, int PASS = 0>
 __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
 {
-    const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t lin = blk * blockDim.x + threadIdx.x;
+    const uint32_t nfb = (PASS == 3) ? face_blocks(a) : 0u;
+    const bool face_wg = (PASS == 2) || (PASS == 3 && blockIdx.x < nfb);
     uint32_t icu;
-    if constexpr (PASS == 2) {
+    if (face_wg) {
+        const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
         if (lin >= a.nface) return;
         icu = a.fcells[lin];
     } else {
+        const uint32_t g = blockIdx.x - nfb;
+        const uint32_t blk = (a.tune & TUNE_NO_XCD) ? g : xcd_block(g, gridDim.x - nfb);
+        const uint32_t lin = blk * blockDim.x + threadIdx.x;
         if (lin >= a.N) return;
         icu = (a.tune & TUNE_YTILE) ? ytile_cell(a, lin) : lin;
     }
@@ -700,8 +719,10 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
 #define CELL_IBB IBB
 #define CELL_WALE_INTERIOR true
 #define CELL_AA AA
-#define CELL_SKIP_FACE (PASS == 1)
+#define CELL_SKIP_FACE (PASS == 1 || (PASS == 3 && !face_wg))
+#define CELL_PURE_OK (AA == 0 && !face_wg)
 #include "d3q19_cell.inc"
+#undef CELL_PURE_OK
 #undef CELL_SKIP_FACE
 #undef CELL_AA
 #undef CELL_WALE_INTERIOR

@@ -531,25 +531,32 @@ constexpr int kStep27MinWaves = sizeof(Real) == 8 ? 1 : 7;
 template <typename Real>
 constexpr int kBatch27MinWaves = sizeof(Real) == 8 ? 1 : 6;
 
-// PASS as step19_kernel: 1 the cells off the lattice faces, 2 the listed face cells
+// PASS as step19_kernel: 1 the cells off the lattice faces, 2 the listed face cells, 3 both
+// in one grid (the face-list work-groups first)
 template <typename S, typename Real, int SGS, int PASS = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStep27MinWaves<Real>, 8)))
 step27_kernel(const StepArgs<Real, 27> a)
 {
-    const uint32_t blk = (a.tune & TUNE_NO_XCD) ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t lin = blk * blockDim.x + threadIdx.x;
+    const uint32_t nfb = (PASS == 3) ? face_blocks(a) : 0u;
+    const bool face_wg = (PASS == 2) || (PASS == 3 && blockIdx.x < nfb);
     uint32_t icu;
-    if constexpr (PASS == 2) {
+    if (face_wg) {
+        const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
         if (lin >= a.nface) return;
         icu = a.fcells[lin];
     } else {
+        const uint32_t g = blockIdx.x - nfb;
+        const uint32_t blk = (a.tune & TUNE_NO_XCD) ? g : xcd_block(g, gridDim.x - nfb);
+        const uint32_t lin = blk * blockDim.x + threadIdx.x;
         if (lin >= a.N) return;
         icu = (a.tune & TUNE_YTILE) ? ytile_cell(a, lin) : lin;
     }
 #define CELL_RETURN return
 #define CELL_WALE_INTERIOR true
-#define CELL_SKIP_FACE (PASS == 1)
+#define CELL_SKIP_FACE (PASS == 1 || (PASS == 3 && !face_wg))
+#define CELL_PURE_OK (!face_wg)
 #include "d3q27_cell.inc"
+#undef CELL_PURE_OK
 #undef CELL_SKIP_FACE
 #undef CELL_WALE_INTERIOR
 #undef CELL_RETURN

@@ -17,7 +17,7 @@
 #include "d3q19_step.hpp"
 #include "d3q27_step.hpp"
 #include "ibm.hpp"
-#include "batch.hpp"
+#include "launch.hpp"
 #include "geom.hpp"
 
 using namespace lbffs;
@@ -66,6 +66,9 @@ struct lbffs_ctx {
     int aa_odd = 0;  // in-place streaming: the array is in the layout of a first half-step
     uint32_t *fcells = nullptr;  // non-solid face cells (the face pass), rebuilt with the flags
     uint32_t nface = 0;
+    uint32_t *wpure = nullptr;   // per-wave summary of the flags (wave_pure), rebuilt with them
+    uint64_t flags_gen = 0;      // bumped on every rebuild of the flags (a batch keeps its own copy)
+    uint32_t nstep = 0;          // steps run (the step number of the stochastic rounding)
     int32_t *boundary = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
     uint8_t *solid = nullptr, *neiSolid = nullptr;
     void *sdf = nullptr, *u0 = nullptr, *v0 = nullptr, *w0 = nullptr, *omega = nullptr;
@@ -452,8 +455,22 @@ int pack_flags(lbffs_ctx *c)
             HIP_TRY(hipMemcpy(c->fcells, cells.data(), cells.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
             c->nface = (uint32_t)cells.size();
         }
+        // per-wave summary: bit w set when cells 64w .. 64w + 63 all exist, are fluid and lie
+        // off the lattice faces (their pulls are the fixed interior offsets)
+        const size_t nw = (size_t)c->N / 64;
+        std::vector<uint32_t> bits((nw + 31) / 32 + 1, 0u);
+        for (size_t w = 0; w < nw; w++) {
+            bool pure = true;
+            for (size_t l = 0; l < 64 && pure; l++) pure = (fl[w * 64 + l] & (F_SOLID | F_FACE)) == 0;
+            if (pure) bits[w >> 5] |= 1u << (w & 31);
+        }
+        if (!c->wpure) {
+            if (int e = dalloc((void **)&c->wpure, bits.size() * sizeof(uint32_t))) return e;
+        }
+        HIP_TRY(hipMemcpy(c->wpure, bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     c->flags_dirty = false;
+    c->flags_gen++;
     return 0;
 }
 
@@ -516,6 +533,8 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
     a.tune = c->tune;
     a.fcells = c->fcells;
     a.nface = c->nface;
+    a.step = c->nstep;
+    a.step_ctr = nullptr;
     {
         const uint32_t ty = ytile_rows(c);
         const uint32_t t = ty ? ty : (uint32_t)ny;
@@ -526,42 +545,10 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
         a.dplane = make_fastdiv(a.tile_plane);
         if (!ty) a.tune &= ~TUNE_YTILE;
     }
-}
-
-// aa: 0 two-lattice streaming, 1 or 2 the first or second in-place half-step (D3Q19).
-// With TUNE_SPLIT a two-lattice step runs in two passes: the cells off the lattice faces,
-// then the face cells of the list (a.fcells); otherwise, and for in-place half-steps, one.
-template <typename S, typename Real, int Q, int SGS>
-void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream_t st, int aa)
-{
-    const bool split = (a.tune & TUNE_SPLIT) && aa == 0;
-    const unsigned fgrid = (a.nface + (unsigned)block - 1) / (unsigned)block;
-    if constexpr (Q == 27) {
-        if (split) {
-            hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 1>), dim3(grid), dim3(block), 0, st, a);
-            if (fgrid) hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 2>), dim3(fgrid), dim3(block), 0, st, a);
-        } else {
-            hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 0>), dim3(grid), dim3(block), 0, st, a);
-        }
-    } else {
-        if (aa == 1)
-            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 1>), dim3(grid), dim3(block), 0, st, a);
-        else if (aa == 2)
-            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 2>), dim3(grid), dim3(block), 0, st, a);
-        else if (a.ibb && split) {
-            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true, 0, 1>), dim3(grid), dim3(block), 0, st, a);
-            if (fgrid)
-                hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true, 0, 2>), dim3(fgrid), dim3(block), 0, st, a);
-        } else if (a.ibb) {
-            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true, 0, 0>), dim3(grid), dim3(block), 0, st, a);
-        } else if (split) {
-            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 0, 1>), dim3(grid), dim3(block), 0, st, a);
-            if (fgrid)
-                hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 0, 2>), dim3(fgrid), dim3(block), 0, st, a);
-        } else {
-            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 0, 0>), dim3(grid), dim3(block), 0, st, a);
-        }
-    }
+    // the summary names aligned runs of 64 cells: a wave covers one when its cells are
+    // consecutive (the plain order, or y-tiles whose rows are whole waves)
+    const bool runs = !(a.tune & TUNE_YTILE) || nx % 64 == 0;
+    a.wpure = (runs && !(c->tune & TUNE_NO_PURE) && c->d.streaming != LBFFS_STREAM_AA) ? c->wpure : nullptr;
 }
 
 // In-place streaming: bring the array back to the two-lattice layout (after an odd number
@@ -686,6 +673,7 @@ int run_steps_q(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
         // read the velocities of the marker boxes and the density of the cells they force
         a.mac_all = (a.aux_all || c->d.sgs == LBFFS_SGS_WALE || ibm) ? 1 : 0;
         a.rho_all = (a.aux_all || ibm) ? 1 : 0;
+        a.step = c->nstep;
         const int phase = aa ? 1 + c->aa_odd : 0;
         switch (c->d.sgs) {
         case LBFFS_SGS_WALE: launch_step<S, Real, Q, 0>(a, grid, block, c->stream, phase); break;
@@ -698,6 +686,7 @@ int run_steps_q(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
         else
             c->cur ^= 1;
         c->mcur ^= 1;
+        c->nstep++;
         if (ibm) {
             if (int e = ibm_after_step<S, Real, Q>(c, last)) return e;
         }
@@ -855,7 +844,7 @@ void free_all(lbffs_ctx *c)
     void *ptrs[] = {c->f[0], c->f[1], c->boundary, c->b1, c->b2, c->b3, c->solid, c->neiSolid, c->sdf, c->u0, c->v0,
                     c->w0, c->omega, c->Fwx, c->Fwy, c->Fwz, c->tau, c->Gx, c->Gy, c->Gz, c->rho[0], c->rho[1],
                     c->u[0], c->u[1], c->v[0], c->v[1], c->w[0], c->w[1], c->flag, c->d_err, c->d_part, c->d_red,
-                    c->markers, c->Gm, c->mark, c->list, c->count, c->qlink, c->fcells};
+                    c->markers, c->Gm, c->mark, c->list, c->count, c->qlink, c->fcells, c->wpure};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -878,6 +867,8 @@ struct lbffs_batch {
     uint32_t nfblocks = 0;
     uint32_t *fcells = nullptr;
     bool ftable_ok = false;
+    std::vector<uint64_t> gen;  // flags generation of each lattice the face table was built from
+    uint32_t *ctr = nullptr;    // device counter of the steps run in the current call
     int grid = 0;        // work-groups of the launch; 0 = one per table entry
     int graph_chunk = 0; // steps per captured hipGraph; 0 = plain launches
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -888,6 +879,13 @@ struct lbffs_batch {
 };
 
 namespace {
+
+// the batch's counter of steps within a call: reset at the start of a call, advanced after
+// every step (a kernel, so that a captured graph advances it on every replay)
+__global__ void step_ctr_kernel(uint32_t *ctr, int reset)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *ctr = reset ? 0u : *ctr + 1u;
+}
 
 void batch_drop_graphs(lbffs_batch *b)
 {
@@ -902,14 +900,9 @@ template <typename S, typename Real, int Q, int SGS, bool IBB>
 void batch_launch(lbffs_batch *b, int parity, int last)
 {
     const StepArgs<Real, Q> *args = (const StepArgs<Real, Q> *)b->args_dev[parity + 2 * last];
-    const unsigned grid = b->grid > 0 ? (unsigned)b->grid : b->nblocks;
-    hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, IBB, 1>), dim3(grid), dim3(kBatchBlock), 0, b->stream, args,
-                       (const BatchBlock *)b->table, b->nblocks, (const uint32_t *)nullptr);
-    if (b->nfblocks) {
-        const unsigned fgrid = b->grid > 0 ? std::min((unsigned)b->grid, b->nfblocks) : b->nfblocks;
-        hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, IBB, 2>), dim3(fgrid), dim3(kBatchBlock), 0, b->stream,
-                           args, (const BatchBlock *)b->ftable, b->nfblocks, (const uint32_t *)b->fcells);
-    }
+    launch_batch<S, Real, Q, SGS, IBB>(args, (const BatchBlock *)b->table, b->nblocks, (const BatchBlock *)b->ftable,
+                                       b->nfblocks, (const uint32_t *)b->fcells, b->grid > 0 ? (unsigned)b->grid : 0u,
+                                       b->stream);
 }
 
 // the face table: every non-solid face cell of every lattice, 256 to a block
@@ -942,6 +935,8 @@ int batch_face_table(lbffs_batch *b)
         b->nfblocks = (uint32_t)blocks.size();
     }
     batch_drop_graphs(b);  // the captured launches name the old table
+    b->gen.resize(b->ctx.size());
+    for (size_t d = 0; d < b->ctx.size(); d++) b->gen[d] = b->ctx[d]->flags_gen;
     b->ftable_ok = true;
     return 0;
 }
@@ -960,7 +955,10 @@ int batch_steps_sgs(lbffs_batch *b, int nsteps, int cur0)
                 b->gexec[p] = nullptr;
                 hipGraph_t g = nullptr;
                 HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-                for (int k = 0; k < chunk; k++) batch_launch<S, Real, Q, SGS, IBB>(b, (p + k) & 1, 0);
+                for (int k = 0; k < chunk; k++) {
+                    batch_launch<S, Real, Q, SGS, IBB>(b, (p + k) & 1, 0);
+                    hipLaunchKernelGGL(step_ctr_kernel, dim3(1), dim3(64), 0, b->stream, b->ctr, 0);
+                }
                 HIP_TRY(hipStreamEndCapture(b->stream, &g));
                 hipError_t e = hipGraphInstantiate(&b->gexec[p], g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
@@ -975,6 +973,7 @@ int batch_steps_sgs(lbffs_batch *b, int nsteps, int cur0)
     for (; s < nsteps; s++) {
         const int p = (cur0 + s) & 1;
         batch_launch<S, Real, Q, SGS, IBB>(b, p, s == nsteps - 1 ? 1 : 0);
+        hipLaunchKernelGGL(step_ctr_kernel, dim3(1), dim3(64), 0, b->stream, b->ctr, 0);
         HIP_TRY(hipGetLastError());
     }
     return 0;
@@ -1005,9 +1004,12 @@ int batch_steps_q(lbffs_batch *b, int nsteps, const lbffs_step_params *params)
             a.aux_all = (params[d].store_every || last) ? 1 : 0;
             a.mac_all = (a.aux_all || c->d.sgs == LBFFS_SGS_WALE) ? 1 : 0;
             a.rho_all = a.aux_all;
+            a.step = c->nstep;  // + the device counter of steps within this call
+            a.step_ctr = b->ctr;
         }
         HIP_TRY(hipMemcpy(b->args_dev[v], b->args_host[v].data(), b->args_host[v].size(), hipMemcpyHostToDevice));
     }
+    hipLaunchKernelGGL(step_ctr_kernel, dim3(1), dim3(64), 0, b->stream, b->ctr, 1);
     // D3Q19's interpolated bounce-back is a kernel variant (its loop costs registers in
     // every cell); D3Q27 selects the q source per link at run time
     bool ibb = false;
@@ -1057,7 +1059,7 @@ void batch_free(lbffs_batch *b)
         delete c;
     }
     for (void *p : {b->args_dev[0], b->args_dev[1], b->args_dev[2], b->args_dev[3], (void *)b->table,
-                    (void *)b->ftable, (void *)b->fcells})
+                    (void *)b->ftable, (void *)b->fcells, (void *)b->ctr})
         if (p) (void)hipFree(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -1242,14 +1244,19 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
         }
     }
     const int cur0 = c->cur, mcur0 = c->mcur, block0 = c->block, tune0 = c->tune;
+    const uint32_t nstep0 = c->nstep;
     int rc = 0;
     hipError_t he = hipMemcpyAsync(saved_f, c->f[cur0], fbytes, hipMemcpyDeviceToDevice, c->stream);
     for (int k = 0; k < 11 && he == hipSuccess; k++)
         he = hipMemcpyAsync((char *)saved_r + k * n * rs, keep[k], n * rs, hipMemcpyDeviceToDevice, c->stream);
     if (saved_m && he == hipSuccess) he = hipMemcpyAsync(saved_m, c->markers, mbytes, hipMemcpyDeviceToDevice, c->stream);
     const int blocks[3] = {64, 128, 256};
-    int tunes[6] = {0, TUNE_NO_XCD, TUNE_SPLIT, TUNE_SPLIT | TUNE_NO_XCD, TUNE_YTILE, TUNE_SPLIT | TUNE_YTILE};
-    int ntunes = ytile_rows(c) ? 6 : 4;
+    // launch options: the work-group order (XCD slabs or the hardware's), the face cells in
+    // the same waves as their neighbours, in a second launch or in the first work-groups of
+    // the one launch, and the y-tiled cell order where the lattice allows it
+    int tunes[9] = {0, TUNE_NO_XCD, TUNE_FACE_FIRST, TUNE_FACE_FIRST | TUNE_NO_XCD, TUNE_SPLIT,
+                    TUNE_SPLIT | TUNE_NO_XCD, TUNE_YTILE, TUNE_FACE_FIRST | TUNE_YTILE, TUNE_SPLIT | TUNE_YTILE};
+    int ntunes = ytile_rows(c) ? 9 : 6;
     if (c->d.streaming == LBFFS_STREAM_AA) {  // one pass per half-step
         tunes[2] = TUNE_YTILE;
         ntunes = ytile_rows(c) ? 3 : 2;
@@ -1281,6 +1288,7 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
     // restore the state
     c->cur = cur0;
     c->mcur = mcur0;
+    c->nstep = nstep0;
     c->aa_odd = 0;
     if (he == hipSuccess) he = hipMemcpyAsync(c->f[cur0], saved_f, fbytes, hipMemcpyDeviceToDevice, c->stream);
     for (int k = 0; k < 11 && he == hipSuccess; k++)
@@ -1314,7 +1322,7 @@ int lbffs_set_launch(lbffs_ctx *c, int block, int tune)
 {
     if (!c) return fail(LBFFS_E_INVALID, "null argument");
     if (block % 64 != 0 || block < 64 || block > 256) return fail(LBFFS_E_INVALID, "block must be 64, 128, 192 or 256");
-    if (tune & ~(TUNE_NT_STORE | TUNE_NO_XCD | TUNE_YTILE | TUNE_SPLIT))
+    if (tune & ~(TUNE_NT_STORE | TUNE_NO_XCD | TUNE_YTILE | TUNE_SPLIT | TUNE_NO_PURE | TUNE_FACE_FIRST))
         return fail(LBFFS_E_INVALID, "unknown launch option");
     c->block = block;
     c->tune = tune;
@@ -1494,6 +1502,7 @@ int lbffs_batch_create(const lbffs_desc *descs, int n, lbffs_batch **out)
     const size_t amax = sizeof(StepArgs<double, 27>) * (size_t)n;
     int e = 0;
     for (int v = 0; v < 4 && !e; v++) e = dalloc(&b->args_dev[v], amax);
+    if (!e) e = dalloc((void **)&b->ctr, sizeof(uint32_t));
     if (!e) e = dalloc((void **)&b->table, table.size() * sizeof(BatchBlock));
     if (!e && hipMemcpy(b->table, table.data(), table.size() * sizeof(BatchBlock), hipMemcpyHostToDevice) != hipSuccess)
         e = fail(LBFFS_E_DEVICE, "batch table upload");
@@ -1541,8 +1550,10 @@ int lbffs_batch_step(lbffs_batch *b, int nsteps, const lbffs_step_params *params
         if (c->flags_dirty) {
             int e = is_f64(c) ? pack_flags<double>(c) : pack_flags<float>(c);
             if (e) return fail(e, "lattice " + std::to_string(d) + ": " + g_last_error);
-            b->ftable_ok = false;
         }
+        // the flags may also have been rebuilt by a call on the lattice's own context
+        // (lbffs_reduce, lbffs_step, lbffs_autotune): compare generations, not the dirty bit
+        if (b->gen.size() != b->ctx.size() || b->gen[d] != c->flags_gen) b->ftable_ok = false;
     }
     if (!b->ftable_ok) {
         if (int e = batch_face_table(b)) return e;
@@ -1555,6 +1566,7 @@ int lbffs_batch_step(lbffs_batch *b, int nsteps, const lbffs_step_params *params
     for (lbffs_ctx *c : b->ctx) {
         c->cur ^= (nsteps & 1);
         c->mcur ^= (nsteps & 1);
+        c->nstep += (uint32_t)nsteps;
         c->timed = false;
     }
     b->timed = true;

@@ -182,6 +182,8 @@ int lbffs_case_outlet_sponge(int nx, int ny, int nz, int i, int j, int k, int p1
                              float omega0, float *omega)
 {
     if (!omega || spz <= 0) return -1;
+    // the patch cell must lie in the lattice: only the swept coordinate is clamped below
+    if (nx < 1 || ny < 1 || nz < 1 || i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz) return -1;
     const float tau0 = 0.55f;
     if (p1 != 0) {
         for (int s = (int)std::fmax(i - spz, 0); s < std::fmin(i + spz, nx); s++) {

@@ -32,6 +32,20 @@ enum : uint32_t {
 
 __host__ __device__ inline int flag_code(uint32_t fl, int shift) { return (int)((fl >> shift) & 7u); }
 
+// Per-wave summary of the flags: bit w of word w / 32 is set when the 64 cells
+// 64w .. 64w + 63 are all fluid and none lies on a lattice face.  Such a wave pulls from
+// fixed offsets whatever its flags say, so it can issue its pulls together with the flag
+// load instead of one memory round trip after it.  Read through the scalar cache (the
+// wave's first active cell names the run; the caller launches 64-cell aligned waves).
+__device__ inline bool wave_pure(const uint32_t *bits, uint32_t icu)
+{
+    if (!bits) return false;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(icu) >> 6;
+    typedef const __attribute__((address_space(4))) uint32_t const_u32;
+    const uint32_t word = *((const_u32 *)bits + (w >> 5));
+    return ((word >> (w & 31u)) & 1u) != 0;
+}
+
 // BC codes (input.hpp:221-256)
 enum { BC_CYCLIC = 0, BC_BOUNCEBACK = 1, BC_OUTLET = 3, BC_SYMMETRY = 4, BC_FIXEDVELOCITY = 5,
        BC_FIXEDDENSITY = 6, BC_WALLFUNCTION = 7 };
@@ -122,17 +136,32 @@ __device__ inline uint16_t float_to_bf16(float x)
 // 16-bit value removes perturbations below half a unit in the last place every step, and a
 // flow held symmetric by that rounding never leaves it (a cylinder wake at Re 100 does not
 // shed); rounding up with the probability of the remainder keeps the stored value unbiased.
-// The random bits come from a hash of the cell's moments of this step and its index -- a
-// deterministic function of the state, the same in every launch shape, batch and graph
-// replay.
-__device__ inline uint32_t sr_hash(float a, float b, float c, float d, uint32_t ic)
+// The random bits come from a hash of the cell's moments of this step, its index and the
+// step number -- a deterministic function of the state and the step count, the same in
+// every launch shape, batch and graph replay.  The step number makes the bits of a cell
+// whose state no longer changes (a steady flow) differ from step to step, so that its
+// rounding errors average out over time instead of repeating.
+__device__ inline uint32_t sr_hash(float a, float b, float c, float d, uint32_t ic, uint32_t step)
 {
     const uint32_t ub = __float_as_uint(b), uc = __float_as_uint(c), ud = __float_as_uint(d);
     uint32_t h = __float_as_uint(a) ^ ((ub << 11) | (ub >> 21)) ^ ((uc << 19) | (uc >> 13)) ^ ((ud << 27) | (ud >> 5)) ^
                  (ic * 0x9E3779B1u);
     h *= 0x2C1B3C6Du;
     h ^= h >> 15;
+    h ^= step * 0x85EBCA77u;
+    h *= 0xC2B2AE3Du;
+    h ^= h >> 13;
     return h;
+}
+
+// the step number a launch runs: `step` (its own, or the first step of a batched call) plus
+// the batch's device counter of steps within the call (null outside a batch; read through
+// the scalar cache)
+__device__ inline uint32_t sr_step(uint32_t step, const uint32_t *ctr)
+{
+    if (!ctr) return step;
+    typedef const __attribute__((address_space(4))) uint32_t const_u32;
+    return step + *(const_u32 *)ctr;
 }
 
 // the bits of direction q: the cell's hash advanced by q golden-ratio steps, which spreads

@@ -13,7 +13,8 @@ extern "C" {
 
 enum { ORC_JACOBI = 0, ORC_LEX_GS = 1 };
 enum { ORC_SGS_WALE = 0, ORC_SGS_SMAGORINSKY = 1, ORC_SGS_NONE = 2 };
-enum { ORC_COLL_RR = 0, ORC_COLL_BGK = 1 };
+/* D3Q19: RR (the reference's) or BGK; D3Q27: RR (selectable) or CUMULANT (the reference's) */
+enum { ORC_COLL_RR = 0, ORC_COLL_BGK = 1, ORC_COLL_CUMULANT = 2 };
 
 typedef struct {
     int nx, ny, nz;
@@ -76,6 +77,10 @@ void orc27_collide_f32(const float *ft, float rho, float u, float v, float w, fl
                        float clim, float *out);
 void orc27_collide_f64(const double *ft, double rho, double u, double v, double w, double tau, double tauSGS,
                        double omegaB, double clim, double *out);
+void orc27_collide_rr_f32(const float *ft, float rho, float u, float v, float w, float tau, float tauSGS,
+                          float omegaB, float dpdx, float *out);
+void orc27_collide_rr_f64(const double *ft, double rho, double u, double v, double w, double tau, double tauSGS,
+                          double omegaB, double dpdx, double *out);
 
 /* immersed boundary (moving walls); m[10] = trans[3], rot centre[3], rot axis[3], omega */
 long orc_ibm_icbox(long ic, int b, int nx, int ny, int nz);

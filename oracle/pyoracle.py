@@ -22,7 +22,7 @@ REF_HOST_SO = os.path.join(HERE, "_ref", "libref_host.so")
 
 JACOBI, LEX_GS = 0, 1
 SGS_WALE, SGS_SMAGORINSKY, SGS_NONE = 0, 1, 2
-COLL_RR, COLL_BGK = 0, 1
+COLL_RR, COLL_BGK, COLL_CUMULANT = 0, 1, 2
 
 
 class OrcParams(C.Structure):
@@ -60,6 +60,8 @@ def _bind(L):
             getattr(L, f"orc{lat}_step_{p}").restype = C.c_int
     L.orc27_collide_f32.argtypes = [vp] + [C.c_float] * 8 + [vp]
     L.orc27_collide_f64.argtypes = [vp] + [C.c_double] * 8 + [vp]
+    L.orc27_collide_rr_f32.argtypes = [vp] + [C.c_float] * 8 + [vp]
+    L.orc27_collide_rr_f64.argtypes = [vp] + [C.c_double] * 8 + [vp]
     L.orc27_upwind_B.argtypes = [C.c_int] * 10
     L.orc27_upwind_B.restype = C.c_long
     L.orc27_mirror.argtypes = [C.c_int, C.c_int]
@@ -155,7 +157,17 @@ def initial_state(case, real=np.float32):
     return st
 
 
-def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=None, sgs=None, fma=False,
+def case_collision(case):
+    """The oracle's collision code for a case: the case's own choice (a product LBFFS_COLL_*
+    value: 0 recursive regularized, 1 cumulant) or, when it names none, the reference's
+    collision of its lattice (RR on D3Q19, K19:3637; cumulant on D3Q27, K27:2824)."""
+    cc = getattr(case, "collision", None)
+    if cc is None:
+        return COLL_CUMULANT if getattr(case, "q", 19) == 27 else COLL_RR
+    return COLL_RR if cc == 0 else COLL_CUMULANT
+
+
+def run19(case, nsteps, order=JACOBI, precision="f32", collision=None, state=None, sgs=None, fma=False,
           ibm=None):
     """Run nsteps of the restated reference step (D3Q19, or D3Q27 when the case has
     q == 27).  Returns the state dict (f = current populations after the swaps, plus the
@@ -167,6 +179,8 @@ def run19(case, nsteps, order=JACOBI, precision="f32", collision=COLL_RR, state=
     a = case.arrays
     st = initial_state(case, real) if state is None else {k: v.copy() for k, v in state.items()}
     p = case.params
+    if collision is None:
+        collision = case_collision(case)
     P = OrcParams(nx=case.nx, ny=case.ny, nz=case.nz, order=order, sgs=case.sgs if sgs is None else sgs,
                   collision=collision, damping=int(p.get("damping", 1)), dpdx=p.get("dpdx", 0.0),
                   LES=p.get("LES", 0.0), omegaB=p.get("omegaB", 0.1), spzWidth=p.get("spzWidth", 0.1),
@@ -215,6 +229,16 @@ def collide27(ft, rho, u, v, w, tau, tauSGS=0.0, omegaB=1.0, clim=0.01, precisio
     a = np.ascontiguousarray(ft, real)
     out = np.empty(27, real)
     getattr(orc(), f"orc27_collide_{precision}")(_p(a), rho, u, v, w, tau, tauSGS, omegaB, clim, _p(out))
+    return out
+
+
+def collide27_rr(ft, rho, u, v, w, tau, tauSGS=0.0, omegaB=1.0, dpdx=0.0, precision="f64"):
+    """The D3Q27 recursive-regularized collision (u: the velocity without the half-step
+    force shift, as the D3Q19 collision is called, K19:3641)."""
+    real = np.float64 if precision == "f64" else np.float32
+    a = np.ascontiguousarray(ft, real)
+    out = np.empty(27, real)
+    getattr(orc(), f"orc27_collide_rr_{precision}")(_p(a), rho, u, v, w, tau, tauSGS, omegaB, dpdx, _p(out))
     return out
 
 

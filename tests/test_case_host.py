@@ -104,3 +104,19 @@ def test_outlet_patch_sponge_shape():
     tau = fx * (1 / om0) + (1 - fx) * 0.55
     assert np.allclose(ramp[1:], (1 / tau)[1:], rtol=1e-6)
     assert abs(row[-1] - 1 / 0.55) < 1e-6
+
+
+def test_outlet_patch_sponge_rejects_cells_outside_the_lattice():
+    """The patch cell (i, j, k) must lie in the lattice: only the swept coordinate is
+    clamped, so an index outside it would write omega out of bounds.  The array is left
+    untouched when the call is refused."""
+    nx, ny, nz, spz = 12, 4, 3, 3
+    om0 = np.float32(1.5)
+    L = lb.lib()
+    for (i, j, k) in ((-1, 1, 1), (nx, 1, 1), (5, -1, 1), (5, ny, 1), (5, 1, -1), (5, 1, nz)):
+        omega = np.full(nx * ny * nz, om0, np.float32)
+        assert L.lbffs_case_outlet_sponge(nx, ny, nz, i, j, k, 3, 3, 3, spz, om0, lb._ptr(omega)) == -1
+        assert np.all(omega == om0)
+    omega = np.full(nx * ny * nz, om0, np.float32)
+    assert L.lbffs_case_outlet_sponge(0, ny, nz, 0, 0, 0, 3, 0, 0, spz, om0, lb._ptr(omega)) == -1
+    assert L.lbffs_case_outlet_sponge(nx, ny, nz, nx - 1, 1, 1, 3, 0, 0, spz, om0, lb._ptr(omega)) == 0

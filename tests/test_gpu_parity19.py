@@ -112,11 +112,22 @@ def test_block_size_invariance(block):
 
 
 @pytest.mark.parametrize("sgs", [lb.SGS_SMAGORINSKY, lb.SGS_WALE])
-@pytest.mark.parametrize("tune", [1, 2, 3, 4, 6, 8, 10, 12])
+@pytest.mark.parametrize("tune", [1, 2, 3, 4, 6, 8, 10, 12, 16, 32, 34, 36, 48])
 def test_launch_options_bitwise_identical(tune, sgs):
     # non-temporal stores, the work-group order, the y-tiled cell order (4: tiles of 16
-    # rows here) and the split face pass (8) change no arithmetic
+    # rows here), the split face pass (8), the per-wave flag summary switched off (16) and
+    # the face cells in the first work-groups of the one launch (32) change no arithmetic
     c = cases.cylinder_channel(nx=64, ny=32, nz=4, D=8, cx=20, cy=16, spz_cells=6, sgs=sgs)
+    a = run_hip(c, 25)
+    b = run_hip(c, 25, tune=tune)
+    for k in ("f", "u", "tauSGS", "Fwx"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("tune", [8, 32, 34])
+def test_launch_options_bitwise_identical_ibb(tune):
+    # the interpolated bounce-back variant of the kernel under the face-pass launch options
+    c = cases.cylinder_channel(nx=64, ny=32, nz=4, D=8, cx=20, cy=16, spz_cells=6, ibb=True)
     a = run_hip(c, 25)
     b = run_hip(c, 25, tune=tune)
     for k in ("f", "u", "tauSGS", "Fwx"):
@@ -160,7 +171,7 @@ def test_reference_kernel_cavity_short():
     e_rh = compare(h, r, keys=("rho", "vel", "f"), tol=1e-3, label="hip vs refcl")
 
 
-@pytest.mark.parametrize("tune", [8, 10, 12])
+@pytest.mark.parametrize("tune", [8, 10, 12, 16, 32, 34])
 def test_launch_options_bitwise_identical_16bit(tune):
     # stochastic rounding draws its bits from the cell's own state: the split face pass and
     # the y-tiled order leave the FP16 results bit-identical too

@@ -94,7 +94,7 @@ def test_store_modes_bitwise_identical27():
         assert np.array_equal(a[k], d[k]), k
 
 
-@pytest.mark.parametrize("tune", [1, 2, 3, 4, 6, 8, 10, 12])
+@pytest.mark.parametrize("tune", [1, 2, 3, 4, 6, 8, 10, 12, 16, 32, 34, 36])
 def test_launch_options_bitwise_identical27(tune):
     c = cases.cavity(16, q=27)
     a = run_hip(c, 20)
