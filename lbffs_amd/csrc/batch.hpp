@@ -31,7 +31,7 @@ constexpr int kBatchBlock = 256;
 // wave holds a face cell (i = 0 or nx - 1), and a wave with both kinds runs the face path
 // and the interior pull: split in two launches, each wave runs one of them.  Both passes
 // read only the previous step's arrays, so their order does not matter.
-template <typename S, typename Real, int SGS, int Q, bool IBB, int PASS>
+template <typename S, typename Real, int SGS, int Q, bool IBB, int PASS, int COLL = COLL_CUMULANT>
 __global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Q == 27 ? kBatch27MinWaves<Real> : 1, 8)))
     batch_step_kernel(const StepArgs<Real, Q> *__restrict__ args, const BatchBlock *__restrict__ table,
                       uint32_t nblocks, const uint32_t *__restrict__ cells)
@@ -54,11 +54,13 @@ __global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_e
 #define CELL_AA 0
 #define CELL_SKIP_FACE (PASS == 1)
 #define CELL_PURE_OK (PASS == 1)
+#define CELL_COLL COLL
             if constexpr (Q == 27) {
 #include "d3q27_cell.inc"
             } else {
 #include "d3q19_cell.inc"
             }
+#undef CELL_COLL
 #undef CELL_PURE_OK
 #undef CELL_SKIP_FACE
 #undef CELL_AA

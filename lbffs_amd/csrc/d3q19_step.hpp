@@ -71,6 +71,7 @@ struct StepArgs {
     const uint32_t *fcells;  // the lattice's non-solid face cells (the face pass runs these)
     uint32_t nface;
     const uint32_t *wpure;   // per-wave summary (wave_pure), or null: every wave reads its flags first
+    int coll;                 // D3Q27: LBFFS_COLL_RR or LBFFS_COLL_CUMULANT (uniform per launch)
     uint32_t step;            // step number (stochastic rounding), see sr_step
     const uint32_t *step_ctr; // batches: device counter of the steps run in the current call
 };
@@ -390,6 +391,40 @@ __device__ inline Mom19<Real> moments19(const Real *ft)
     m.sxx = a12 + p78 + p910 + p1112 + p1314;
     m.syy = a34 + p78 + p910 + p1516 + p1718;
     m.szz = a56 + p1112 + p1314 + p1516 + p1718;
+    m.sxy = p78 - p910;
+    m.sxz = p1112 - p1314;
+    m.syz = p1516 - p1718;
+    return m;
+}
+
+// The same moments for populations held as 16-bit deviations from the weights: density
+// and second moments are formed from the deviations f_q - w_q (exact differences, the
+// populations being within a factor of two of their weights) plus the exact weight moments
+// (sum w_q = 1, sum w_q c_a c_b = delta_ab / 3).  Summed as w_q + deviation, the density
+// would carry the FP32 weights' excess sum_q fl(w_q) - 1 (about 2 ulp of 1) below every
+// deviation a 16-bit value can hold, and its rounding to FP32 would go the same way in
+// every cell and every step: a closed lattice would gain mass at about 1e-7 per cell per
+// step, whatever the 16-bit rounding.  The momentum sums pairwise differences, as above.
+template <typename Real>
+__device__ inline Mom19<Real> moments19_dev(const Real *ft)
+{
+    Real d[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) d[q] = ft[q] - w19<Real>(q);
+    Mom19<Real> m;
+    const Real a12 = d[1] + d[2], a34 = d[3] + d[4], a56 = d[5] + d[6];
+    const Real p78 = d[7] + d[8], p910 = d[9] + d[10], p1112 = d[11] + d[12], p1314 = d[13] + d[14];
+    const Real p1516 = d[15] + d[16], p1718 = d[17] + d[18];
+    m.rho = Real(1) + (d[0] + a12 + a34 + a56 + p78 + p910 + p1112 + p1314 + p1516 + p1718);
+    const Real d78 = d[7] - d[8], d910 = d[9] - d[10], d1112 = d[11] - d[12], d1314 = d[13] - d[14];
+    const Real d1516 = d[15] - d[16], d1718 = d[17] - d[18];
+    m.mx = (d[1] - d[2]) + d78 + d910 + d1112 + d1314;
+    m.my = (d[3] - d[4]) + d78 - d910 + d1516 + d1718;
+    m.mz = (d[5] - d[6]) + d1112 - d1314 + d1516 - d1718;
+    const Real third = Real(1) / Real(3);
+    m.sxx = third + (a12 + p78 + p910 + p1112 + p1314);
+    m.syy = third + (a34 + p78 + p910 + p1516 + p1718);
+    m.szz = third + (a56 + p1112 + p1314 + p1516 + p1718);
     m.sxy = p78 - p910;
     m.sxz = p1112 - p1314;
     m.syz = p1516 - p1718;

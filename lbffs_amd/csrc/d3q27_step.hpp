@@ -522,6 +522,108 @@ __device__ inline void cumulant27(Real *ft, Real *K, Real rho, Real u, Real v, R
     for (int q = 0; q < 27; q++) ft[q] = K[L27::slot[q]];
 }
 
+// The recursive-regularized collision on the D3Q27 lattice, selectable at the D3Q27 call
+// site (K27:2824): the structure of the reference's D3Q19 collisionRecursiveRegularized
+// (K19:3156-3309) over the full D3Q27 Hermite set.  The second-order non-equilibrium
+// moments come from the cell's shared raw moments; orders three to six are rebuilt
+// recursively from them (each coefficient the sum, over the unordered index pairs of its
+// multi-index, of N_pair times the velocities of the other indices -- the rule K19:3215-3224
+// applies to x^2 y and x^2 y^2) and relaxed at omegaEff towards u^a v^b w^c; the second
+// order splits its trace (omegaB) from the deviator (omegaEff) as K19:3226-3231.  Hermite
+// coefficients become raw moments per axis (x^2 = H_xx + 1/3), and three passes of the
+// 1-D inverse transform (f(-1), f(0), f(+1)) = ((m2 - m1)/2, m0 - m2, (m2 + m1)/2) return
+// the populations.  u is the force-shifted velocity minus the half-step shift (K19:3641);
+// the body force enters as S_q = rho w_q 3 c_qx dpdx (K19:3278-3287).  K is scratch.
+template <typename Real>
+__device__ inline void collide_rr27(Real *ft, Real *K, const Mom19<Real> &m, Real inv_rho, Real u, Real v, Real w,
+                                    Real omegaEff, Real omegaB, Real dpdx)
+{
+    const Real cs2 = Real(1) / Real(3), cs4 = Real(1) / Real(9), cs6 = Real(1) / Real(27);
+    const Real rho = m.rho;
+    const Real R200 = (m.sxx - cs2 * rho) * inv_rho, R020 = (m.syy - cs2 * rho) * inv_rho;
+    const Real R002 = (m.szz - cs2 * rho) * inv_rho;
+    const Real R110 = m.sxy * inv_rho, R101 = m.sxz * inv_rho, R011 = m.syz * inv_rho;
+    const Real uu = u * u, vv = v * v, ww = w * w, uv = u * v, uw = u * w, vw = v * w;
+    const Real N200 = R200 - uu, N020 = R020 - vv, N002 = R002 - ww;
+    const Real N110 = R110 - uv, N101 = R101 - uw, N011 = R011 - vw;
+    const Real omegaM = (omegaB - omegaEff) * cs2, omegaP = omegaM + omegaEff, om1 = Real(1) - omegaEff;
+    const Real C200 = R200 - omegaP * N200 - omegaM * N020 - omegaM * N002;
+    const Real C020 = R020 - omegaM * N200 - omegaP * N020 - omegaM * N002;
+    const Real C002 = R002 - omegaM * N200 - omegaM * N020 - omegaP * N002;
+    const Real C110 = R110 - omegaEff * N110, C101 = R101 - omegaEff * N101, C011 = R011 - omegaEff * N011;
+    // third order
+    const Real C210 = uu * v + om1 * (v * N200 + Real(2) * u * N110);
+    const Real C201 = uu * w + om1 * (w * N200 + Real(2) * u * N101);
+    const Real C021 = vv * w + om1 * (w * N020 + Real(2) * v * N011);
+    const Real C120 = vv * u + om1 * (u * N020 + Real(2) * v * N110);
+    const Real C102 = ww * u + om1 * (u * N002 + Real(2) * w * N101);
+    const Real C012 = ww * v + om1 * (v * N002 + Real(2) * w * N011);
+    const Real C111 = uv * w + om1 * (w * N110 + v * N101 + u * N011);
+    // fourth order
+    const Real C220 = uu * vv + om1 * (vv * N200 + uu * N020 + Real(4) * uv * N110);
+    const Real C202 = uu * ww + om1 * (ww * N200 + uu * N002 + Real(4) * uw * N101);
+    const Real C022 = vv * ww + om1 * (ww * N020 + vv * N002 + Real(4) * vw * N011);
+    const Real C211 = uu * vw + om1 * (vw * N200 + Real(2) * (uw * N110 + uv * N101) + uu * N011);
+    const Real C121 = vv * uw + om1 * (uw * N020 + Real(2) * (vw * N110 + uv * N011) + vv * N101);
+    const Real C112 = ww * uv + om1 * (uv * N002 + Real(2) * (vw * N101 + uw * N011) + ww * N110);
+    // fifth and sixth order
+    const Real uvw = uv * w;
+    const Real C221 = uu * vv * w + om1 * (vv * w * N200 + uu * w * N020 + Real(4) * uvw * N110
+                                           + Real(2) * (u * vv * N101 + uu * v * N011));
+    const Real C212 = uu * v * ww + om1 * (v * ww * N200 + uu * v * N002 + Real(4) * uvw * N101
+                                           + Real(2) * (u * ww * N110 + uu * w * N011));
+    const Real C122 = u * vv * ww + om1 * (u * ww * N020 + u * vv * N002 + Real(4) * uvw * N011
+                                           + Real(2) * (v * ww * N110 + vv * w * N101));
+    const Real C222 = uu * vv * ww + om1 * (vv * ww * N200 + uu * ww * N020 + uu * vv * N002
+                                            + Real(4) * (uvw * w * N110 + uvw * v * N101 + uvw * u * N011));
+    // raw moments per unit density in the transform's layout K[9 a + 3 b + c] = M_abc
+    K[0] = Real(1);
+    K[9] = u;
+    K[3] = v;
+    K[1] = w;
+    K[18] = C200 + cs2;
+    K[6] = C020 + cs2;
+    K[2] = C002 + cs2;
+    K[12] = C110;
+    K[10] = C101;
+    K[4] = C011;
+    K[21] = C210 + cs2 * v;
+    K[19] = C201 + cs2 * w;
+    K[7] = C021 + cs2 * w;
+    K[15] = C120 + cs2 * u;
+    K[11] = C102 + cs2 * u;
+    K[5] = C012 + cs2 * v;
+    K[13] = C111;
+    K[24] = C220 + cs2 * (C200 + C020) + cs4;
+    K[20] = C202 + cs2 * (C200 + C002) + cs4;
+    K[8] = C022 + cs2 * (C020 + C002) + cs4;
+    K[22] = C211 + cs2 * C011;
+    K[16] = C121 + cs2 * C101;
+    K[14] = C112 + cs2 * C110;
+    K[25] = C221 + cs2 * (C201 + C021) + cs4 * w;
+    K[23] = C212 + cs2 * (C210 + C012) + cs4 * v;
+    K[17] = C122 + cs2 * (C120 + C102) + cs4 * u;
+    K[26] = C222 + cs2 * (C220 + C202 + C022) + cs4 * (C200 + C020 + C002) + cs6;
+    // back to populations: x, y, z passes of the 1-D inverse transform (no shift)
+    auto inv3 = [](Real &a, Real &b, Real &c) {
+        const Real m0 = a, h1 = Real(0.5) * b, h2 = Real(0.5) * c;
+        a = h2 - h1;
+        b = m0 - c;
+        c = h2 + h1;
+    };
+#pragma unroll
+    for (int p = 0; p < 9; p++) inv3(K[p], K[9 + p], K[18 + p]);
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) inv3(K[9 * x + c], K[9 * x + 3 + c], K[9 * x + 6 + c]);
+#pragma unroll
+    for (int p = 0; p < 9; p++) inv3(K[3 * p], K[3 * p + 1], K[3 * p + 2]);
+    const Real F = rho * Real(3) * dpdx;
+#pragma unroll
+    for (int q = 0; q < 27; q++) ft[q] = rho * K[L27::slot[q]] + F * w27<Real>(q) * Real(L27::cx[q]);
+}
+
 // At least seven waves per SIMD (72 VGPRs) for the 32-bit arithmetic: the cumulant update
 // is VALU bound and fits without spilling (eight waves, 64 VGPRs, would spill); the
 // compiler's own choice went as low as five.  The FP64 form keeps the compiler's choice.
@@ -533,7 +635,7 @@ constexpr int kBatch27MinWaves = sizeof(Real) == 8 ? 1 : 6;
 
 // PASS as step19_kernel: 1 the cells off the lattice faces, 2 the listed face cells, 3 both
 // in one grid (the face-list work-groups first)
-template <typename S, typename Real, int SGS, int PASS = 0>
+template <typename S, typename Real, int SGS, int PASS = 0, int COLL = COLL_CUMULANT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStep27MinWaves<Real>, 8)))
 step27_kernel(const StepArgs<Real, 27> a)
 {
@@ -554,8 +656,11 @@ step27_kernel(const StepArgs<Real, 27> a)
 #define CELL_RETURN return
 #define CELL_WALE_INTERIOR true
 #define CELL_SKIP_FACE (PASS == 1 || (PASS == 3 && !face_wg))
-#define CELL_PURE_OK (!face_wg)
+// (the 16-bit split interior pass would spill with the per-wave fast path)
+#define CELL_PURE_OK (!face_wg && !(PASS == 1 && sizeof(S) == 2))
+#define CELL_COLL COLL
 #include "d3q27_cell.inc"
+#undef CELL_COLL
 #undef CELL_PURE_OK
 #undef CELL_SKIP_FACE
 #undef CELL_WALE_INTERIOR

@@ -11,6 +11,20 @@
 
 namespace lbffs {
 
+template <typename S, typename Real, int SGS, int COLL>
+void launch_step27(const StepArgs<Real, 27> &a, unsigned grid, unsigned fgrid, int block, hipStream_t st, bool merged,
+                   bool split)
+{
+    if (merged) {
+        hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 3, COLL>), dim3(fgrid + grid), dim3(block), 0, st, a);
+    } else if (split) {
+        hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 1, COLL>), dim3(grid), dim3(block), 0, st, a);
+        if (fgrid) hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 2, COLL>), dim3(fgrid), dim3(block), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 0, COLL>), dim3(grid), dim3(block), 0, st, a);
+    }
+}
+
 template <typename S, typename Real, int Q, int SGS>
 void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream_t st, int aa)
 {
@@ -18,14 +32,10 @@ void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream
     const bool split = (a.tune & TUNE_SPLIT) && aa == 0 && !merged;
     const unsigned fgrid = (a.nface + (unsigned)block - 1) / (unsigned)block;
     if constexpr (Q == 27) {
-        if (merged) {
-            hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 3>), dim3(fgrid + grid), dim3(block), 0, st, a);
-        } else if (split) {
-            hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 1>), dim3(grid), dim3(block), 0, st, a);
-            if (fgrid) hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 2>), dim3(fgrid), dim3(block), 0, st, a);
-        } else {
-            hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 0>), dim3(grid), dim3(block), 0, st, a);
-        }
+        if (a.coll == COLL_RR)
+            launch_step27<S, Real, SGS, COLL_RR>(a, grid, fgrid, block, st, merged, split);
+        else
+            launch_step27<S, Real, SGS, COLL_CUMULANT>(a, grid, fgrid, block, st, merged, split);
     } else {
         if (aa == 1)
             hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 1>), dim3(grid), dim3(block), 0, st, a);
@@ -53,16 +63,25 @@ void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream
 
 template <typename S, typename Real, int Q, int SGS, bool IBB>
 void launch_batch(const StepArgs<Real, Q> *args, const BatchBlock *table, uint32_t nblocks, const BatchBlock *ftable,
-                  uint32_t nfblocks, const uint32_t *fcells, unsigned grid, hipStream_t st)
+                  uint32_t nfblocks, const uint32_t *fcells, unsigned grid, int coll, hipStream_t st)
 {
     const unsigned g = grid > 0 ? grid : nblocks;
+    const unsigned fg = grid > 0 ? (grid < nfblocks ? grid : nfblocks) : nfblocks;
+    if constexpr (Q == 27) {
+        if (coll == COLL_RR) {
+            hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, IBB, 1, COLL_RR>), dim3(g), dim3(kBatchBlock), 0,
+                               st, args, table, nblocks, (const uint32_t *)nullptr);
+            if (nfblocks)
+                hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, IBB, 2, COLL_RR>), dim3(fg), dim3(kBatchBlock),
+                                   0, st, args, ftable, nfblocks, fcells);
+            return;
+        }
+    }
     hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, IBB, 1>), dim3(g), dim3(kBatchBlock), 0, st, args, table,
                        nblocks, (const uint32_t *)nullptr);
-    if (nfblocks) {
-        const unsigned fg = grid > 0 ? (grid < nfblocks ? grid : nfblocks) : nfblocks;
+    if (nfblocks)
         hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, IBB, 2>), dim3(fg), dim3(kBatchBlock), 0, st, args,
                            ftable, nfblocks, fcells);
-    }
 }
 
 #if KERN_STORE == 0
@@ -87,7 +106,7 @@ template void launch_step<KS, KR, KERN_Q, 2>(const StepArgs<KR, KERN_Q> &, unsig
 #define LBFFS_BATCH_INST(SGS, IBB)                                                                                  \
     template void launch_batch<KS, KR, KERN_Q, SGS, IBB>(const StepArgs<KR, KERN_Q> *, const BatchBlock *, uint32_t, \
                                                          const BatchBlock *, uint32_t, const uint32_t *, unsigned,   \
-                                                         hipStream_t);
+                                                         int, hipStream_t);
 LBFFS_BATCH_INST(0, false)
 LBFFS_BATCH_INST(1, false)
 LBFFS_BATCH_INST(2, false)

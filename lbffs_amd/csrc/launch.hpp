@@ -18,8 +18,9 @@ void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream
 
 // One batched step: the block table (cells off the lattice faces), then the face table.
 // grid = 0: one work-group per table entry; otherwise a grid-stride walk with that many.
+// coll: the lattices' collision (COLL_RR or COLL_CUMULANT; D3Q19 runs RR whatever it says).
 template <typename S, typename Real, int Q, int SGS, bool IBB>
 void launch_batch(const StepArgs<Real, Q> *args, const BatchBlock *table, uint32_t nblocks, const BatchBlock *ftable,
-                  uint32_t nfblocks, const uint32_t *fcells, unsigned grid, hipStream_t st);
+                  uint32_t nfblocks, const uint32_t *fcells, unsigned grid, int coll, hipStream_t st);
 
 }  // namespace lbffs

@@ -108,6 +108,8 @@ namespace {
 constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
 
+
+
 // ---------------------------------------------------------------- kernels
 
 template <typename Real>
@@ -266,10 +268,13 @@ __global__ void __launch_bounds__(kRedThreads) reduce_kernel(uint32_t N, size_t 
         acc[3] += (double)Gx[ic];
         acc[4] += (double)Gy[ic];
         acc[5] += (double)Gz[ic];
-        Real rho = 0;
+        // summed in double: each decoded population is exact, and an FP32 sum near 1 would
+        // round every cell's density alike (the weights' FP32 excess lies below its ulp)
+        double rho = 0;
 #pragma unroll
-        for (int q = 0; q < Q; q++) rho += Codec<S, Real, Q>::load(f + (size_t)q * fs + ic, q, wq<Real, Q>(q));
-        acc[6] += (double)rho - 1.0;
+        for (int q = 0; q < Q; q++)
+            rho += (double)Codec<S, Real, Q>::load(f + (size_t)q * fs + ic, q, wq<Real, Q>(q));
+        acc[6] += rho - 1.0;
         acc[7] += 1.0;
     }
     __shared__ double sh[kRedThreads / 64][LBFFS_REDUCE_N];
@@ -535,6 +540,7 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
     a.nface = c->nface;
     a.step = c->nstep;
     a.step_ctr = nullptr;
+    a.coll = c->d.collision;
     {
         const uint32_t ty = ytile_rows(c);
         const uint32_t t = ty ? ty : (uint32_t)ny;
@@ -761,23 +767,24 @@ int download_f(lbffs_ctx *c, void *host)
     return 0;
 }
 
+// part: kRedBlocks x LBFFS_REDUCE_N partial sums; red: the LBFFS_REDUCE_N final sums
 template <typename S, typename Real>
-int reduce_impl(lbffs_ctx *c, double *out)
+int reduce_impl(lbffs_ctx *c, double *out, double *part, double *red)
 {
     if (c->d.q == 27)
         hipLaunchKernelGGL((reduce_kernel<S, Real, 27>), dim3(kRedBlocks), dim3(kRedThreads), 0, c->stream,
                            (uint32_t)c->N, c->fstride, c->flag, (const S *)c->f[c->cur], (const Real *)c->Fwx,
                            (const Real *)c->Fwy, (const Real *)c->Fwz, (const Real *)c->Gx, (const Real *)c->Gy,
-                           (const Real *)c->Gz, c->d_part);
+                           (const Real *)c->Gz, part);
     else
         hipLaunchKernelGGL((reduce_kernel<S, Real, 19>), dim3(kRedBlocks), dim3(kRedThreads), 0, c->stream,
                            (uint32_t)c->N, c->fstride, c->flag, (const S *)c->f[c->cur], (const Real *)c->Fwx,
                            (const Real *)c->Fwy, (const Real *)c->Fwz, (const Real *)c->Gx, (const Real *)c->Gy,
-                           (const Real *)c->Gz, c->d_part);
+                           (const Real *)c->Gz, part);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(64), 0, c->stream, c->d_part, kRedBlocks, c->d_red);
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(64), 0, c->stream, (const double *)part, kRedBlocks, red);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(out, c->d_red, sizeof(double) * LBFFS_REDUCE_N, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(out, red, sizeof(double) * LBFFS_REDUCE_N, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -809,7 +816,7 @@ struct DownloadFOp {
 };
 template <typename S, typename Real>
 struct ReduceOp {
-    static int run(lbffs_ctx *c, double *o) { return reduce_impl<S, Real>(c, o); }
+    static int run(lbffs_ctx *c, double *o, double *part, double *red) { return reduce_impl<S, Real>(c, o, part, red); }
 };
 
 template <typename S, typename Real>
@@ -902,7 +909,7 @@ void batch_launch(lbffs_batch *b, int parity, int last)
     const StepArgs<Real, Q> *args = (const StepArgs<Real, Q> *)b->args_dev[parity + 2 * last];
     launch_batch<S, Real, Q, SGS, IBB>(args, (const BatchBlock *)b->table, b->nblocks, (const BatchBlock *)b->ftable,
                                        b->nfblocks, (const uint32_t *)b->fcells, b->grid > 0 ? (unsigned)b->grid : 0u,
-                                       b->stream);
+                                       b->ctx[0]->d.collision, b->stream);
 }
 
 // the face table: every non-solid face cell of every lattice, 256 to a block
@@ -1089,8 +1096,8 @@ int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out)
         return fail(LBFFS_E_UNSUPPORTED, "in-place streaming runs D3Q19 lattices");
     if (desc->q == 19 && desc->collision != LBFFS_COLL_RR)
         return fail(LBFFS_E_UNSUPPORTED, "D3Q19 runs the recursive-regularized collision (K19:3637)");
-    if (desc->q == 27 && desc->collision != LBFFS_COLL_CUMULANT)
-        return fail(LBFFS_E_UNSUPPORTED, "D3Q27 runs the cumulant collision (K27:2824)");
+    if (desc->q == 27 && desc->collision != LBFFS_COLL_CUMULANT && desc->collision != LBFFS_COLL_RR)
+        return fail(LBFFS_E_UNSUPPORTED, "D3Q27 runs the cumulant (K27:2824) or the recursive-regularized collision");
     if (desc->block != 0 && (desc->block % 64 != 0 || desc->block > 256 || desc->block < 64))
         return fail(LBFFS_E_INVALID, "block must be 64, 128, 192 or 256");
     // the step kernels address a population array, and the interior velocity gradient a
@@ -1405,7 +1412,7 @@ int lbffs_reduce(lbffs_ctx *c, double out[LBFFS_REDUCE_N])
         if (e) return e;
     }
     if (int e = aa_restore(c)) return e;  // per-cell density sums in the two-lattice layout
-    return dispatch<ReduceOp>(c, out);
+    return dispatch<ReduceOp>(c, out, c->d_part, c->d_red);
 }
 
 int lbffs_output_macros(lbffs_ctx *c, float dpdx, void *rho, void *u, void *v, void *w, size_t bytes)

@@ -50,6 +50,9 @@ __device__ inline bool wave_pure(const uint32_t *bits, uint32_t icu)
 enum { BC_CYCLIC = 0, BC_BOUNCEBACK = 1, BC_OUTLET = 3, BC_SYMMETRY = 4, BC_FIXEDVELOCITY = 5,
        BC_FIXEDDENSITY = 6, BC_WALLFUNCTION = 7 };
 
+// collision operators (the values of LBFFS_COLL_* in lbffs.h)
+enum { COLL_RR = 0, COLL_CUMULANT = 1 };
+
 // a face code that reflects populations instead of wrapping them (kernel.cl:335-337)
 __host__ __device__ inline bool is_wall_code(int c) { return c == 1 || c == 4 || c == 5 || c == 6 || c == 7; }
 

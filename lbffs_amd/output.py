@@ -106,6 +106,29 @@ def coeffs_fp64(sums, c, L, uMax, Dref):
     return dict(zip(("rho_av", "Cx", "Cy", "Cz", "CxIBM", "CyIBM", "CzIBM"), out[:]))
 
 
+class MeanAccumulator:
+    """The running means of write.hpp:68-74 over n values per component: add(u, v, w, interval)
+    accumulates uMean += u * interval and timeMean += interval in FP32, as the reference
+    host does; mean() returns (uMean, vMean, wMean) / timeMean."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.uMean, self.vMean, self.wMean = (np.zeros(self.n, np.float32) for _ in range(3))
+        self.timeMean = np.zeros(1, np.float32)
+
+    def add(self, u, v, w, interval):
+        a = [np.ascontiguousarray(x, np.float32) for x in (u, v, w)]
+        if any(x.size != self.n for x in a):
+            raise ValueError("MeanAccumulator.add: arrays of the wrong size")
+        _check(lib().lbffs_out_accumulate_mean(self.n, int(interval), _ptr(a[0]), _ptr(a[1]), _ptr(a[2]),
+                                               _ptr(self.uMean), _ptr(self.vMean), _ptr(self.wMean),
+                                               _ptr(self.timeMean)), "lbffs_out_accumulate_mean")
+
+    def mean(self):
+        t = float(self.timeMean[0])
+        return tuple(m.astype(np.float64) / t for m in (self.uMean, self.vMean, self.wMean))
+
+
 class OutputStep:
     """write.hpp around a Lattice: call ``step_begin(nt)`` at the top of every iteration of
     the time loop, as ns.cpp:509-511 includes write.hpp."""

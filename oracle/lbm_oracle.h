@@ -11,7 +11,9 @@
 extern "C" {
 #endif
 
-enum { ORC_JACOBI = 0, ORC_LEX_GS = 1 };
+/* ORC_LEX_WAVE: the results of ORC_LEX_GS computed in parallel (D3Q19, without the
+ * interpolated bounce-back, whose diagonal neighbour reads it cannot order) */
+enum { ORC_JACOBI = 0, ORC_LEX_GS = 1, ORC_LEX_WAVE = 2 };
 enum { ORC_SGS_WALE = 0, ORC_SGS_SMAGORINSKY = 1, ORC_SGS_NONE = 2 };
 /* D3Q19: RR (the reference's) or BGK; D3Q27: RR (selectable) or CUMULANT (the reference's) */
 enum { ORC_COLL_RR = 0, ORC_COLL_BGK = 1, ORC_COLL_CUMULANT = 2 };

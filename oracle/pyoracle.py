@@ -20,7 +20,7 @@ REF_CO19 = os.path.join(HERE, "_ref", "kernel19.co")
 REF_CO27 = os.path.join(HERE, "_ref", "kernel27.co")
 REF_HOST_SO = os.path.join(HERE, "_ref", "libref_host.so")
 
-JACOBI, LEX_GS = 0, 1
+JACOBI, LEX_GS, LEX_WAVE = 0, 1, 2  # LEX_WAVE: LEX_GS results, computed in parallel (D3Q19)
 SGS_WALE, SGS_SMAGORINSKY, SGS_NONE = 0, 1, 2
 COLL_RR, COLL_BGK, COLL_CUMULANT = 0, 1, 2
 

@@ -57,7 +57,8 @@ int run_steps(int Q, const char *co_path, int nx, int ny, int nz, int nsteps, in
 {
     const long N = (long)nx * ny * nz;
     if (nx < 2 || ny < 2 || nz < 1 || N <= 0 || (long)Q * N >= (1L << 31)) return err("lattice size");
-    if (block <= 0 || block > 1024 || N % block != 0) return err("N must be a multiple of the work-group size");
+    // the kernels are built without a work-group size attribute: at most 256 work-items each
+    if (block <= 0 || block > 256 || N % block != 0) return err("work-group size above 256, or not dividing N");
     // inputs that would send the reference kernel outside its buffers
     for (long ic = 0; ic < N; ic++) {
         const int i = (int)(ic % ((long)nx * ny) % nx), j = (int)(ic % ((long)nx * ny) / nx),

@@ -1,5 +1,8 @@
 """Shared helpers of the GPU parity tests: run a case through the C ABI and compare it
 with the oracle (same Jacobi neighbour-read order) in FP32 and FP64."""
+import json
+import os
+
 import numpy as np
 
 import lbffs_amd as lb
@@ -16,7 +19,7 @@ def _ibm_motion_kw(m):
 
 def run_hip(case, nsteps, storage=lb.STORE_F32, store_every=0, block=0, chunks=1, tune=None, autotune=False,
             ibm=None):
-    coll = lb.COLL_CUMULANT if case.q == 27 else lb.COLL_RR
+    coll = case.coll
     with lb.Lattice(case.nx, case.ny, case.nz, q=case.q, storage=storage, sgs=case.sgs, collision=coll,
                     omega0=case.omega0, block=block) as lat:
         case.upload_to(lat)
@@ -73,7 +76,16 @@ def _ulp_perturbed_state(case, seed):
     return st
 
 
-def parity(case, nsteps, keys=("rho", "vel", "f"), tau_tol=1e-4, label="", **kw):
+def _log(label, record):
+    """Append the measured errors and floors of a parity check to $LBFFS_PARITY_LOG (a JSON
+    lines file), so that the per-case numbers of a run can be kept with it."""
+    path = os.environ.get("LBFFS_PARITY_LOG")
+    if path:
+        with open(path, "a") as fh:
+            fh.write(json.dumps({"case": label, **record}) + "\n")
+
+
+def parity(case, nsteps, keys=("rho", "vel", "f"), tau_tol=1e-4, label="", h=None, **kw):
     """HIP FP32 against the oracle in FP32 and FP64 (same Jacobi read order).
 
     floor = the FP32 rounding level of the case: the largest distance from the FP64 run
@@ -86,7 +98,8 @@ def parity(case, nsteps, keys=("rho", "vel", "f"), tau_tol=1e-4, label="", **kw)
     tauSGS (a derived, strongly nonlinear field) is held to tau_tol against oracle32.
     """
     ibm = kw.get("ibm")
-    h = run_hip(case, nsteps, **kw)
+    if h is None:
+        h = run_hip(case, nsteps, **kw)
     o32 = po.run19(case, nsteps, order=po.JACOBI, ibm=ibm)
     o64 = po.run19(case, nsteps, order=po.JACOBI, precision="f64", ibm=ibm)
     e32 = _errs(h, o32, keys)
@@ -102,6 +115,7 @@ def parity(case, nsteps, keys=("rho", "vel", "f"), tau_tol=1e-4, label="", **kw)
             fl = {k: max(fl[k], fp[k]) for k in keys}
     print(label, "hip-o32", {k: f"{v:.2e}" for k, v in e32.items()}, "hip-o64",
           {k: f"{v:.2e}" for k, v in e64.items()}, "floor", {k: f"{v:.2e}" for k, v in fl.items()}, flush=True)
+    _log(label, {"steps": nsteps, "hip_o32": e32, "hip_o64": e64, "floor": fl})
     for k in keys:
         assert e32[k] <= max(TOL, 2 * fl[k]), f"{label} {k}: |hip-o32| {e32[k]:.3e} (floor {fl[k]:.3e})"
         assert e64[k] <= 2 * fl[k] + 1e-6, f"{label} {k}: |hip-o64| {e64[k]:.3e} (floor {fl[k]:.3e})"

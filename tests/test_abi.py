@@ -44,8 +44,9 @@ def test_strerror_and_invalid_arguments():
     assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -1
     d = lb.Desc(abi_version=lb.ABI_VERSION, q=19, nx=4, ny=4, nz=4, storage=9)
     assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -1
-    # lattice/collision pairs of the reference kernels only (K19:3637 RR, K27:2824 cumulant)
-    for q, coll in ((27, lb.COLL_RR), (19, lb.COLL_CUMULANT), (21, lb.COLL_RR)):
+    # lattice/collision pairs: D3Q19 runs the RR collision (K19:3637); D3Q27 the cumulant
+    # (K27:2824) or the RR collision on the D3Q27 Hermite set; D3Q19 has no cumulant
+    for q, coll in ((19, lb.COLL_CUMULANT), (21, lb.COLL_RR), (27, 7)):
         d = lb.Desc(abi_version=lb.ABI_VERSION, q=q, nx=4, ny=4, nz=4, collision=coll)
         assert L.lbffs_create(ctypes.byref(d), ctypes.byref(h)) == -4
     assert L.lbffs_ibm_set_markers(None, None, 0) == -1

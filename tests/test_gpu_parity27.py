@@ -208,3 +208,60 @@ def test_bluff_body27_qlinks_vs_oracle():
     fw_o = np.array([o[k].astype(np.float64).sum() for k in ("Fwx", "Fwy", "Fwz")])
     assert fw_h[0] > 0
     assert rel_l2(fw_h, fw_o) <= 1e-4, (fw_h, fw_o)
+
+
+# ---------------------------------------------------------------- D3Q27 recursive regularized
+# The RR collision on the D3Q27 Hermite set (K19:3156-3309 carried over, selectable at the
+# D3Q27 call site K27:2824) against its oracle restatement, same bounds as above.
+
+@pytest.mark.parametrize("sgs", [lb.SGS_WALE, lb.SGS_SMAGORINSKY, lb.SGS_NONE])
+def test_cavity27_rr_vs_oracle(sgs):
+    c = cases.cavity(16, sgs=sgs, q=27)
+    c.collision = lb.COLL_RR
+    c.params["LES"] = 1.0
+    parity(c, 80, label=f"cavity16 q27 RR sgs={sgs}")
+
+
+def test_channel27_rr_force_vs_oracle():
+    # body force through S_q and the half-step velocity shift, as the D3Q19 RR takes it
+    c = cases.channel(nx=8, ny=20, nz=4, tau=0.8, force=1e-5, q=27)
+    c.collision = lb.COLL_RR
+    parity(c, 200, label="channel q27 RR")
+
+
+@pytest.mark.parametrize("ibb", [False, True])
+def test_cylinder27_rr_vs_oracle(ibb):
+    # inlet with noise, outlet + sponge, interpolated bounce-back (q from the distances, or
+    # precomputed analytic q-distances)
+    c = cases.cylinder_channel(nx=64, ny=32, nz=6, D=10, cx=20, cy=16, Re=60.0, spz_cells=6, q=27,
+                               sdf_from_geometry=not ibb, ibb=ibb)
+    c.collision = lb.COLL_RR
+    h, o, _ = parity(c, 100, label=f"cylinder q27 RR ibb={ibb}")
+    fw_h = np.array([h["Fwx"].astype(np.float64).sum(), h["Fwy"].astype(np.float64).sum()])
+    fw_o = np.array([o["Fwx"].astype(np.float64).sum(), o["Fwy"].astype(np.float64).sum()])
+    assert rel_l2(fw_h, fw_o) <= 1e-4, (fw_h, fw_o)
+
+
+@pytest.mark.parametrize("tune", [8, 32])
+def test_rr27_launch_options_bitwise_identical(tune):
+    c = cases.cavity(16, q=27)
+    c.collision = lb.COLL_RR
+    a = run_hip(c, 20)
+    b = run_hip(c, 20, tune=tune, block=128)
+    assert np.array_equal(a["f"], b["f"])
+    # and it is not the cumulant run
+    c.collision = None
+    d = run_hip(c, 20)
+    assert not np.array_equal(a["f"], d["f"])
+
+
+@pytest.mark.parametrize("storage", [lb.STORE_F16DEV, lb.STORE_BF16DEV])
+def test_rr27_16bit_storage_accuracy(storage):
+    c = cases.cylinder_channel(nx=96, ny=48, nz=8, D=12, cx=30, cy=24, Re=60.0, spz_cells=8, sgs=lb.SGS_SMAGORINSKY,
+                               q=27)
+    c.collision = lb.COLL_RR
+    h = run_hip(c, 300, storage=storage)
+    d = run_hip(c, 300, storage=lb.STORE_F64)
+    e = rel_l2(_vel(h), _vel(d))
+    print("RR27 16-bit storage", storage, f"velocity rel L2 vs f64 {e:.2e}", flush=True)
+    assert e <= (1e-3 if storage == lb.STORE_F16DEV else 2e-2)

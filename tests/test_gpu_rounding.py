@@ -120,18 +120,32 @@ def test_stochastic_rounding_is_unbiased(q, storage):
             lat.step(1, **c.params)
             y = lat.download("f").reshape(q, -1)[:, inner]
         m = y.shape[1]
+        # The 16-bit forms take their moments from the deviations, the FP32 storage form
+        # from the populations: the two steps round differently at the level of an FP32
+        # ulp of the weight, so the value the 16-bit form rounds is within a few such ulps
+        # (scaled to the stored deviation) of d.  Where d lies that close to a 16-bit value
+        # only the number of stored values is checked.
         for k in range(q):
             is_lo, is_hi = y[k] == dec_lo[k], y[k] == dec_hi[k]
-            assert (is_lo | is_hi).all(), (rnd, k)
+            tol = 8 * float(np.spacing(w[k])) * float(scale[k])
+            near_grid = min(abs(d[k] - lo[k]), abs(hi[k] - d[k])) <= tol
+            if not near_grid:
+                assert (is_lo | is_hi).all(), (rnd, k)
+            else:
+                vals = np.unique(y[k])
+                assert len(vals) <= 2, (rnd, k, vals)
+                continue
             if rnd == lb.ROUND_NEAREST:
                 near = dec_hi[k] if abs(hi[k] - d[k]) < abs(d[k] - lo[k]) else dec_lo[k]
-                if abs(hi[k] - d[k]) != abs(d[k] - lo[k]):
+                mid_gap = abs(abs(hi[k] - d[k]) - abs(d[k] - lo[k]))
+                if mid_gap > 2 * tol:
                     assert (y[k] == near).all(), k
                 continue
             p = 0.0 if hi[k] == lo[k] else float((d[k] - lo[k]) / (hi[k] - lo[k]))
             share = float(is_hi.mean()) if hi[k] != lo[k] else 0.0
             sigma = np.sqrt(max(p * (1 - p), 1e-12) / m)
-            assert abs(share - p) < 6 * sigma + 1e-9, (k, share, p)
+            dp = 0.0 if hi[k] == lo[k] else tol / float(hi[k] - lo[k])
+            assert abs(share - p) < 6 * sigma + dp + 1e-9, (k, share, p)
         if rnd == lb.ROUND_STOCHASTIC:
             print(f"q{q} storage {storage}: {m} cells, rounded-up shares vs positions ok", flush=True)
 

@@ -310,3 +310,55 @@ def test_read_orders_differ_by_less_than_tolerance_near_steady_state():
     va = np.concatenate([a["u"], a["v"], a["w"]])
     vb = np.concatenate([b["u"], b["v"], b["w"]])
     assert 0 < rel_l2(va, vb) < 1e-5
+
+
+def _mixed_faces_case():
+    """FixedVelocity, FixedDensity, WallFunction, Symmetry and BounceBack faces with WALE:
+    every kind of neighbour read the face conditions make."""
+    codes = {f: ("Cyclic", "Cyclic", "Cyclic") for f in cases.FACES}
+    codes["nxMin"] = ("FixedVelocity", "Cyclic", "Cyclic")
+    codes["nxMax"] = ("FixedDensity", "Cyclic", "Cyclic")
+    codes["nyMin"] = ("Cyclic", "WallFunction", "Cyclic")
+    codes["nyMax"] = ("Cyclic", "Symmetry", "Cyclic")
+    codes["nzMin"] = ("Cyclic", "Cyclic", "BounceBack")
+    codes["nzMax"] = ("Cyclic", "Cyclic", "BounceBack")
+    return cases.make_case("mixed", 14, 10, 8, 1.0, 1.0, 0.05, 0.002, codes, {"nxMin": (1.0, 0.0, 0.0)},
+                           sgs=lb.SGS_WALE)
+
+
+@pytest.mark.parametrize("which", ["cavity", "cavity2d", "mixed", "outlet", "cylinder"])
+def test_wavefront_order_equals_lexicographic_order(which):
+    """ORC_LEX_WAVE (hyperplanes h = i + 2j + 4k in order, each in parallel) reproduces
+    the sequential in-place lexicographic order bit for bit: the cells a cell reads in
+    place are its face neighbours, periodic wraps included."""
+    if which == "cavity":
+        c = cases.cavity(12, sgs=lb.SGS_WALE)
+    elif which == "cavity2d":
+        c = cases.cavity(16, nz=1, sgs=lb.SGS_WALE)
+    elif which == "mixed":
+        c = _mixed_faces_case()
+    elif which == "outlet":
+        c = cases.cylinder_channel(nx=40, ny=20, nz=6, D=6, cx=12, cy=10, Re=60.0, spz_cells=5, sgs=lb.SGS_WALE)
+        c.arrays["solid"][:] = 0
+        c.arrays["neiSolid"][:] = 0
+    else:
+        c = cases.cylinder_channel(nx=40, ny=20, nz=6, D=6, cx=12, cy=10, Re=60.0, spz_cells=5,
+                                   sgs=lb.SGS_SMAGORINSKY)
+    for precision in ("f32", "f64"):
+        a = po.run19(c, 30, order=po.LEX_GS, precision=precision)
+        b = po.run19(c, 30, order=po.LEX_WAVE, precision=precision)
+        j = po.run19(c, 30, order=po.JACOBI, precision=precision)
+        for k in ("f", "rho", "u", "v", "w", "tauSGS", "Fwx"):
+            assert np.array_equal(a[k], b[k]), (which, precision, k)
+        # the order matters where cells read neighbours in place (WALE's gradient; with
+        # the local Smagorinsky model and no such face condition both orders agree)
+        if c.sgs == lb.SGS_WALE:
+            assert not np.array_equal(a["u"], j["u"]), which
+        else:
+            assert np.array_equal(a["u"], j["u"]), which
+
+
+def test_wavefront_order_refuses_interpolated_bounce_back():
+    c = cases.cylinder_channel(nx=40, ny=20, nz=4, D=6, cx=12, cy=10, Re=60.0, spz_cells=5, ibb=True)
+    with pytest.raises(RuntimeError):
+        po.run19(c, 1, order=po.LEX_WAVE)
