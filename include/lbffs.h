@@ -172,6 +172,17 @@ int lbffs_download(lbffs_ctx *ctx, int field, void *host, size_t bytes);
 #define LBFFS_REDUCE_N 8
 int lbffs_reduce(lbffs_ctx *ctx, double out[LBFFS_REDUCE_N]);
 
+/* No counterpart in the reference: lbffs_reduce for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same sums, bit for bit, into the same out[8]; nothing is allocated or freed.  The
+ * reduction's partial sums live in a fixed device array of the library (1024 x 8 + 8
+ * doubles, 64 KiB, one per GPU, held until the process ends), whatever the lattice size.
+ * Every call on one GPU shares that array: calls that overlap on one GPU race.  Returns
+ * LBFFS_E_STATE while the lattice's flags are to be rebuilt (after an upload of a field
+ * they derive from; lbffs_step, lbffs_reduce or lbffs_autotune rebuilds them, and that
+ * rebuild allocates). */
+int lbffs_reduce_noalloc(lbffs_ctx *ctx, double out[LBFFS_REDUCE_N]);
+
 /* Density and velocity of every cell from the current populations, as the host computes
  * them for an output step (cal_rhoUVW, D3Q19_opencl/D3Q19.hpp:647-666 and
  * D3Q27_opencl/D3Q27.hpp:28-47, called at write.hpp:27): sums in direction order,

@@ -91,6 +91,7 @@ def lib():
         L.lbffs_download.argtypes = [vp, ip, vp, sz]
         L.lbffs_step.argtypes = [vp, ip, C.POINTER(StepParams)]
         L.lbffs_reduce.argtypes = [vp, C.POINTER(C.c_double)]
+        L.lbffs_reduce_noalloc.argtypes = [vp, C.POINTER(C.c_double)]
         L.lbffs_synchronize.argtypes = [vp]
         L.lbffs_last_step_ms.argtypes = [vp, C.POINTER(C.c_float)]
         L.lbffs_destroy.argtypes = [vp]
@@ -132,7 +133,7 @@ def lib():
         L.lbffs_out_stats_compute.argtypes = [vp, vp]
         L.lbffs_out_coeffs_fp64.argtypes = [vp, fl, fl, fl, fl, vp]
         L.lbffs_out_accumulate_mean.argtypes = [C.c_long, ip, vp, vp, vp, vp, vp, vp, vp]
-        for n in ("lbffs_create", "lbffs_upload", "lbffs_download", "lbffs_step", "lbffs_reduce",
+        for n in ("lbffs_create", "lbffs_upload", "lbffs_download", "lbffs_step", "lbffs_reduce", "lbffs_reduce_noalloc",
                   "lbffs_synchronize", "lbffs_last_step_ms", "lbffs_destroy", "lbffs_case_units",
                   "lbffs_case_faces", "lbffs_case_init_feq", "lbffs_case_neisolid", "lbffs_case_outlet_sponge",
                   "lbffs_case_find_points",

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <mutex>
 
 #include "../../include/lbffs.h"
 #include "../../include/lbffs_geom.h"
@@ -68,6 +69,7 @@ struct lbffs_ctx {
     uint32_t nface = 0;
     uint32_t *wpure = nullptr;   // per-wave summary of the flags (wave_pure), rebuilt with them
     uint64_t flags_gen = 0;      // bumped on every rebuild of the flags (a batch keeps its own copy)
+    int device = 0;              // the HIP device the context was created on
     uint32_t nstep = 0;          // steps run (the step number of the stochastic rounding)
     int32_t *boundary = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
     uint8_t *solid = nullptr, *neiSolid = nullptr;
@@ -108,7 +110,20 @@ namespace {
 constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
 
+}  // namespace
 
+// Scratch of lbffs_reduce_noalloc: the per-block partial sums of the FP64 reduction
+// (kRedBlocks x LBFFS_REDUCE_N doubles) followed by the final sums (LBFFS_REDUCE_N), 64 KiB.
+// Each GPU holds its own copy at its own address, kept until the process ends; host code
+// never uses this name as an address -- it uses the pointer hipGetSymbolAddress gives for
+// the device (scratch_ptr below).
+static __device__ __align__(256) double scratch[kRedBlocks * LBFFS_REDUCE_N + LBFFS_REDUCE_N];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices] = {};  // device address of `scratch` on each GPU, set once
+std::mutex g_scratch_mu;
 
 // ---------------------------------------------------------------- kernels
 
@@ -382,6 +397,21 @@ size_t population_stride(size_t n, int elem)
     const size_t bytes = n * (size_t)elem;
     size_t padded = (bytes <= skew) ? skew : ((bytes - skew + span - 1) / span) * span + skew;
     return padded / (size_t)elem;
+}
+
+// the device address of `scratch` on `device` (the current device), looked up on the first
+// call for that device
+int scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(LBFFS_E_UNSUPPORTED, "device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return 0;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(LBFFS_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = (double *)ptr;
+    return 0;
 }
 
 int dalloc(void **p, size_t bytes)
@@ -1111,8 +1141,12 @@ int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out)
                                              " bytes)");
     }
 
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    if (int e = scratch_setup(device)) return e;
     lbffs_ctx *c = new lbffs_ctx();
     c->d = *desc;
+    c->device = device;
     c->N = N;
     c->block = desc->block > 0 ? desc->block : 256;
     c->real_size = (desc->storage == LBFFS_STORE_F64) ? 8 : 4;
@@ -1413,6 +1447,28 @@ int lbffs_reduce(lbffs_ctx *c, double out[LBFFS_REDUCE_N])
     }
     if (int e = aa_restore(c)) return e;  // per-cell density sums in the two-lattice layout
     return dispatch<ReduceOp>(c, out, c->d_part, c->d_red);
+}
+
+int lbffs_reduce_noalloc(lbffs_ctx *c, double out[LBFFS_REDUCE_N])
+{
+    if (!c || !out) return fail(LBFFS_E_INVALID, "null argument");
+    // rebuilding the flags (after an upload that changes them) allocates: refused here
+    if (c->flags_dirty)
+        return fail(LBFFS_E_STATE, "the lattice's flags must be rebuilt first (lbffs_step, lbffs_reduce or "
+                                   "lbffs_autotune does; lbffs_reduce_noalloc allocates nothing)");
+    // the partials fill exactly kRedBlocks rows of the fixed scratch array, whatever the
+    // lattice size (grid-stride kernel); a reduction asking for more rows is refused
+    const long rows = kRedBlocks;
+    if (rows * LBFFS_REDUCE_N + LBFFS_REDUCE_N > (long)(sizeof(scratch) / sizeof(double)))
+        return fail(LBFFS_E_INVALID, "reduction needs more partial rows than the scratch array holds");
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != c->device) HIP_TRY(hipSetDevice(c->device));
+    double *base = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch[c->device] : nullptr;
+    int e = base ? aa_restore(c) : fail(LBFFS_E_STATE, "no scratch address for this device");
+    if (!e) e = dispatch<ReduceOp>(c, out, base, base + rows * LBFFS_REDUCE_N);
+    if (cur != c->device) (void)hipSetDevice(cur);
+    return e;
 }
 
 int lbffs_output_macros(lbffs_ctx *c, float dpdx, void *rho, void *u, void *v, void *w, size_t bytes)
