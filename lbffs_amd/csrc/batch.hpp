@@ -43,6 +43,9 @@ __global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_e
         if constexpr (PASS == 2) {
             if (threadIdx.x >= e.count) continue;
             icu = cells[e.first + threadIdx.x];
+            // a face cell of an XMIX or WRAP wave is updated by the block-table pass
+            const uint32_t cc = cell_wave_class(a.wclass, icu);
+            if (cc == WAVE_XMIX || cc == WAVE_WRAP) continue;
         } else {
             icu = e.first + threadIdx.x;
             if (icu >= a.N) continue;

@@ -70,7 +70,7 @@ struct StepArgs {
     FastDiv dtile, dplane;
     const uint32_t *fcells;  // the lattice's non-solid face cells (the face pass runs these)
     uint32_t nface;
-    const uint32_t *wpure;   // per-wave summary (wave_pure), or null: every wave reads its flags first
+    const uint32_t *wclass;  // per-wave summary (wave_class), or null: every wave reads its flags first
     int coll;                 // D3Q27: LBFFS_COLL_RR or LBFFS_COLL_CUMULANT (uniform per launch)
     uint32_t step;            // step number (stochastic rounding), see sr_step
     const uint32_t *step_ctr; // batches: device counter of the steps run in the current call
@@ -569,9 +569,13 @@ __device__ inline void collide_rr(Real *out, const Mom19<Real> &m, Real inv_rho,
 
 // Boundary handling of a domain-face cell: streaming with walls, then the face
 // conditions in the reference's order (kernel.cl:3581-3592).
+// xonly = false: every direction is streamed here.  xonly = true: a cell on an x face (and
+// on no other face) whose directions were pulled from the interior offsets already; only
+// the directions that cross its face (c_x = +1 at i = 0, c_x = -1 at i = nx - 1) are
+// streamed again.  The face conditions follow in either case.
 template <typename S, typename Real>
 __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, uint32_t fl, long ic, int i, int j, int k,
-                                       Real omega)
+                                       Real omega, bool xonly)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const long FS = a.fstride;
@@ -579,9 +583,11 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
     const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
     const bool sym = (b1 == BC_SYMMETRY || b2 == BC_SYMMETRY || b3 == BC_SYMMETRY);
     Real Fx = 0, Fy = 0, Fz = 0;  // wall momentum of this step's domain-face links
-    ft[0] = Codec<S, Real, 19>::load(fs + ic, 0, w19<Real>(0));
+    const int xs = (i == 0) ? 1 : -1;
+    if (!xonly) ft[0] = Codec<S, Real, 19>::load(fs + ic, 0, w19<Real>(0));
 #pragma unroll
     for (int q = 1; q < 19; q++) {
+        if (xonly && L19::cx[q] != xs) continue;
         long src = face_upwind(q, i, j, k, nx, ny, nz, b1, b2, b3);
         if (src >= 0) {
             ft[q] = Codec<S, Real, 19>::load(fs + (long)q * FS + src, q, w19<Real>(q));
@@ -743,6 +749,9 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
         const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
         if (lin >= a.nface) return;
         icu = a.fcells[lin];
+        // a face cell of an XMIX or WRAP wave is updated by the interior pass
+        const uint32_t cc = cell_wave_class(a.wclass, icu);
+        if (cc == WAVE_XMIX || cc == WAVE_WRAP) return;
     } else {
         const uint32_t g = blockIdx.x - nfb;
         const uint32_t blk = (a.tune & TUNE_NO_XCD) ? g : xcd_block(g, gridDim.x - nfb);

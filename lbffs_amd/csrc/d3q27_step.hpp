@@ -646,6 +646,9 @@ step27_kernel(const StepArgs<Real, 27> a)
         const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
         if (lin >= a.nface) return;
         icu = a.fcells[lin];
+        // a face cell of an XMIX or WRAP wave is updated by the interior pass
+        const uint32_t cc = cell_wave_class(a.wclass, icu);
+        if (cc == WAVE_XMIX || cc == WAVE_WRAP) return;
     } else {
         const uint32_t g = blockIdx.x - nfb;
         const uint32_t blk = (a.tune & TUNE_NO_XCD) ? g : xcd_block(g, gridDim.x - nfb);

@@ -67,7 +67,7 @@ struct lbffs_ctx {
     int aa_odd = 0;  // in-place streaming: the array is in the layout of a first half-step
     uint32_t *fcells = nullptr;  // non-solid face cells (the face pass), rebuilt with the flags
     uint32_t nface = 0;
-    uint32_t *wpure = nullptr;   // per-wave summary of the flags (wave_pure), rebuilt with them
+    uint32_t *wclass = nullptr;  // per-wave summary of the flags (wave_class), rebuilt with them
     uint64_t flags_gen = 0;      // bumped on every rebuild of the flags (a batch keeps its own copy)
     int device = 0;              // the HIP device the context was created on
     uint32_t nstep = 0;          // steps run (the step number of the stochastic rounding)
@@ -431,6 +431,44 @@ int dalloc(void **p, size_t bytes)
     return 0;
 }
 
+// The class of the run of 64 cells from `first` (wave_class): PURE if all are fluid and off
+// the lattice faces; on D3Q19 also XMIX if all are fluid and the face cells among them lie
+// on an x face with j and k interior (so that every cell's interior-offset pulls stay in the
+// lattice), or WRAP if all are fluid cells of one row with i interior, on a y face and/or a z
+// face whose code is Cyclic for every cell (their pulls wrap like interior pulls shifted by
+// whole rows or planes, and no face condition applies); GENERAL otherwise.
+uint32_t wave_class_of(const lbffs_ctx *c, const uint16_t *fl, size_t first)
+{
+    const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
+    bool face = false;
+    for (size_t l = 0; l < 64; l++) {
+        if (fl[first + l] & F_SOLID) return WAVE_GENERAL;
+        face = face || (fl[first + l] & F_FACE);
+    }
+    if (!face) return WAVE_PURE;
+    if (c->d.q != 19) return WAVE_GENERAL;
+    bool xmix = nz >= 3, wrap = true;
+    const size_t i0 = first % (size_t)nx, r0 = first / (size_t)nx;
+    for (size_t l = 0; l < 64; l++) {
+        const size_t ic = first + l;
+        const int i = (int)(ic % (size_t)nx), j = (int)((ic / (size_t)nx) % (size_t)ny);
+        const int k = (int)(ic / ((size_t)nx * (size_t)ny));
+        const uint32_t f = fl[ic];
+        if (f & F_FACE) {
+            const bool xonly = (i == 0 || i == nx - 1) && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2;
+            xmix = xmix && xonly;
+        }
+        // WRAP: one row, i interior, every face it lies on periodic
+        const bool same_row = ic / (size_t)nx == r0 && (l > 0 || i0 == (size_t)i);
+        const bool jface = (j == 0 || j == ny - 1), kface = (k == 0 || k == nz - 1);
+        wrap = wrap && same_row && i >= 1 && i <= nx - 2 && (jface || kface) &&
+               (!jface || flag_code(f, F_B2_SHIFT) == BC_CYCLIC) && (!kface || flag_code(f, F_B3_SHIFT) == BC_CYCLIC);
+    }
+    if (xmix) return WAVE_XMIX;
+    if (wrap) return WAVE_WRAP;
+    return WAVE_GENERAL;
+}
+
 template <typename Real>
 int pack_flags(lbffs_ctx *c)
 {
@@ -490,19 +528,17 @@ int pack_flags(lbffs_ctx *c)
             HIP_TRY(hipMemcpy(c->fcells, cells.data(), cells.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
             c->nface = (uint32_t)cells.size();
         }
-        // per-wave summary: bit w set when cells 64w .. 64w + 63 all exist, are fluid and lie
-        // off the lattice faces (their pulls are the fixed interior offsets)
+        // per-wave summary (wave_class), 2 bits per run of 64 cells
         const size_t nw = (size_t)c->N / 64;
-        std::vector<uint32_t> bits((nw + 31) / 32 + 1, 0u);
+        std::vector<uint32_t> bits((nw + 15) / 16 + 1, 0u);
         for (size_t w = 0; w < nw; w++) {
-            bool pure = true;
-            for (size_t l = 0; l < 64 && pure; l++) pure = (fl[w * 64 + l] & (F_SOLID | F_FACE)) == 0;
-            if (pure) bits[w >> 5] |= 1u << (w & 31);
+            const uint32_t cls = wave_class_of(c, fl.data(), w * 64);
+            bits[w >> 4] |= cls << (2 * (w & 15));
         }
-        if (!c->wpure) {
-            if (int e = dalloc((void **)&c->wpure, bits.size() * sizeof(uint32_t))) return e;
+        if (!c->wclass) {
+            if (int e = dalloc((void **)&c->wclass, bits.size() * sizeof(uint32_t))) return e;
         }
-        HIP_TRY(hipMemcpy(c->wpure, bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(c->wclass, bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     c->flags_dirty = false;
     c->flags_gen++;
@@ -584,7 +620,7 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
     // the summary names aligned runs of 64 cells: a wave covers one when its cells are
     // consecutive (the plain order, or y-tiles whose rows are whole waves)
     const bool runs = !(a.tune & TUNE_YTILE) || nx % 64 == 0;
-    a.wpure = (runs && !(c->tune & TUNE_NO_PURE) && c->d.streaming != LBFFS_STREAM_AA) ? c->wpure : nullptr;
+    a.wclass = (runs && !(c->tune & TUNE_NO_PURE) && c->d.streaming != LBFFS_STREAM_AA) ? c->wclass : nullptr;
 }
 
 // In-place streaming: bring the array back to the two-lattice layout (after an odd number
@@ -881,7 +917,7 @@ void free_all(lbffs_ctx *c)
     void *ptrs[] = {c->f[0], c->f[1], c->boundary, c->b1, c->b2, c->b3, c->solid, c->neiSolid, c->sdf, c->u0, c->v0,
                     c->w0, c->omega, c->Fwx, c->Fwy, c->Fwz, c->tau, c->Gx, c->Gy, c->Gz, c->rho[0], c->rho[1],
                     c->u[0], c->u[1], c->v[0], c->v[1], c->w[0], c->w[1], c->flag, c->d_err, c->d_part, c->d_red,
-                    c->markers, c->Gm, c->mark, c->list, c->count, c->qlink, c->fcells, c->wpure};
+                    c->markers, c->Gm, c->mark, c->list, c->count, c->qlink, c->fcells, c->wclass};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
