@@ -34,6 +34,11 @@ from lbffs_amd import cases  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
 PROFILE_PMC = os.path.join(ROOT, "profiles", "step19_backstep_pmc.txt")
+PROFILE_ENS_PMC = os.path.join(ROOT, "profiles", "batch_ensemble_pmc.txt")
+# bytes one ensemble cell update must move: 19 FP16 populations in and out, and the
+# WALE model's FP32 velocity (its gradient reads the neighbours' velocities of the previous
+# step) written once and read once
+ENS_BYTES = 2 * 19 * 2 + 2 * 3 * 4
 
 
 def parse():
@@ -55,6 +60,9 @@ def parse():
                     help="rounding of the 16-bit storage forms")
     ap.add_argument("--forces-ref", default="f64", choices=["f64", "f32"],
                     help="storage form the other forms' coefficients are compared with")
+    ap.add_argument("--tune-trials", type=int, default=30, help="steps the autotuner times per launch shape")
+    ap.add_argument("--launch", default="",
+                    help="fixed launch shapes instead of the autotuner, e.g. f32=256:2,f16=256:34 (block:options)")
     return ap.parse_args()
 
 
@@ -89,13 +97,19 @@ ROUNDING = {"stochastic": lb.ROUND_STOCHASTIC, "nearest": lb.ROUND_NEAREST}
 STREAMING = {"two-lattice": lb.STREAM_TWO_LATTICE, "aa": lb.STREAM_AA}
 
 
-def run_form(case, storage, warmup, steps, tune=True, rounding=lb.ROUND_STOCHASTIC, streaming=lb.STREAM_TWO_LATTICE):
-    """Warm up, then time exactly `steps` steps bracketed by device synchronisation."""
-    coll = lb.COLL_CUMULANT if case.q == 27 else lb.COLL_RR
-    lat = lb.Lattice(case.nx, case.ny, case.nz, q=case.q, storage=storage, sgs=case.sgs, collision=coll,
+def run_form(case, storage, warmup, steps, tune=True, rounding=lb.ROUND_STOCHASTIC, streaming=lb.STREAM_TWO_LATTICE,
+             trials=30, launch=None):
+    """Warm up, then time exactly `steps` steps bracketed by device synchronisation.  The
+    launch shape is the autotuner's choice (trials steps per candidate) or `launch`
+    (block, options)."""
+    lat = lb.Lattice(case.nx, case.ny, case.nz, q=case.q, storage=storage, sgs=case.sgs, collision=case.coll,
                      omega0=case.omega0, rounding=rounding, streaming=streaming)
     case.upload_to(lat)
-    lat.tuned = lat.autotune(10, **case.params) if tune else None
+    if launch is not None:
+        lat.set_launch(*launch)
+        lat.tuned = {"block": launch[0], "tune": launch[1], "fixed": True}
+    else:
+        lat.tuned = lat.autotune(trials, **case.params) if tune else None
     if warmup:
         lat.step(warmup, **case.params)
     lat.synchronize()
@@ -115,21 +129,35 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def read_pmc():
-    """HBM bytes per launch from the committed counter summary (FETCH_SIZE / WRITE_SIZE
-    of the step kernel, calibrated against the copy probe), or None."""
-    if not os.path.exists(PROFILE_PMC):
+def read_pmc(path=PROFILE_PMC, key="traffic_bytes_per_launch"):
+    """HBM bytes per launch (or per step) from a committed counter summary (read and write
+    request counters of the step kernel, calibrated per request size), or None."""
+    if not os.path.exists(path):
         return None
     vals = {}
-    with open(PROFILE_PMC) as fh:
+    with open(path) as fh:
         for line in fh:
             if "=" in line and not line.lstrip().startswith("#"):
                 k, v = line.split("=", 1)
                 vals[k.strip()] = v.strip()
     try:
-        return float(vals["traffic_bytes_per_launch"])
+        return float(vals[key])
     except (KeyError, ValueError):
         return None
+
+
+def hbm_probes():
+    """Practical HBM rates of plain streaming kernels (GB/s): copy, read-only and write-only
+    over 4 GiB in the launch shapes that reach the highest rates on this part (one float4
+    per work-item, non-temporal hints for copy and read)."""
+    nb = 4 << 30
+    out = {}
+    for kind, name, unroll, nt, block, mult in ((0, "copy", 1, 1, 512, 2), (1, "read", 1, 1, 1024, 1),
+                                                  (2, "write", 1, 0, 256, 1)):
+        ms = lb.probe_stream(kind, unroll, nt, block, 0, nb, reps=10)
+        out[name] = round(mult * nb / (ms * 1e-3) / 1e9, 1)
+    log(f"[bench] HBM probes GB/s: {out}")
+    return out
 
 
 def cpu_baseline(case, nsteps):
@@ -155,10 +183,15 @@ def ensemble_main(a):
     b = lb.Batch([(sp[1], sp[1], sp[1], cases.ensemble_omega0(sp)) for sp in specs], q=19, storage=lb.STORE_F16DEV,
                  sgs=cases.SGS_WALE)
     params = []
+    fluid = 0
+    sample = []  # lattices for the CPU baseline
     for d, sp in enumerate(specs):  # build and upload one lattice at a time
         c = cases.ensemble_case(sp, d)
         c.upload_to(b.lattice[d])
         params.append(c.params)
+        fluid += c.N - int(c.arrays["solid"].sum())
+        if d < 8:
+            sample.append(c)
         if d % 64 == 63:
             log(f"[bench] built and uploaded {d + 1} lattices")
     log(f"[bench] built and uploaded in {time.perf_counter() - t0:.0f} s")
@@ -196,6 +229,22 @@ def ensemble_main(a):
     wall_single = (time.perf_counter() - t1) / ref_steps
     b.close()
     mlups = a.steps * N / wall / 1e6
+    per_step = dev_ms / a.steps * 1e-3
+    traffic = read_pmc(PROFILE_ENS_PMC, "traffic_bytes_per_step")
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle on host cores, the first 8 lattices)")
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po  # noqa: E402
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        for c in sample:
+            po.run19(c, 10, order=po.JACOBI)
+        dt = time.perf_counter() - t0
+        cells = sum(c.N for c in sample)
+        cpu = {"value": 10 * cells / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+               "sample": f"10 steps of the first 8 lattices ({cells} cells), oracle FP32 (the oracle has no "
+                         f"16-bit storage form; Jacobi order, OpenMP {threads} threads), {dt:.1f} s"}
     out = {
         "metric": "aggregate MLUPS of a batched ensemble, D3Q19 FP16-deviation storage",
         "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -204,14 +253,18 @@ def ensemble_main(a):
         "config": {"workload": f"ensemble of {len(specs)} cubic lattices (cavity / cylinder channel, edge 24-96, "
                                f"Re 50-400, seeded)", "cells": N, "storage": "f16",
                    "launch": {"grid": best[1], "graph_chunk": best[2]}, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(N * 76 / (dev_ms / a.steps * 1e-3) / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(fluid * ENS_BYTES / per_step / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(N * 76 / (dev_ms / a.steps * 1e-3) / HBM_PEAK, 4), "traffic": None,
-                     "bytes_per_cell_update": 76, "kernel": "batch_step_kernel"},
+                     "frac": round(fluid * ENS_BYTES / per_step / HBM_PEAK, 4), "traffic": traffic,
+                     "bytes_per_cell_update": ENS_BYTES,
+                     "bytes_per_cell_terms": "19 FP16 populations read and written (76) + WALE's FP32 "
+                                             "velocity, written for the next step and read back (24)",
+                     "fluid_cells": fluid, "kernel": "batch_step_kernel",
+                     "step_ms": round(per_step * 1e3, 4)},
         "one_launch_per_lattice": {"ms_per_step": round(wall_single * 1e3, 3),
                                    "mlups": round(N / wall_single / 1e6, 1),
                                    "batch_speedup": round(wall_single / (wall / a.steps), 2)},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -298,10 +351,16 @@ def main():
     run = [f for f in (a.only.split(",") if a.only else default_forms) if f]
     res = {}
     vel = {}
+    fixed = {}
+    for item in (x for x in a.launch.split(",") if x):
+        k, v = item.split("=")
+        b, t = v.split(":")
+        fixed[k] = (int(b), int(t))
     for name in run:
         log(f"[bench] {name}: warmup {a.warmup}, timed {a.steps}")
         lat, wall, dev_ms = run_form(case, forms[name], a.warmup, a.steps, rounding=ROUNDING[a.rounding],
-                                     streaming=STREAMING[a.streaming])
+                                     streaming=STREAMING[a.streaming], trials=a.tune_trials,
+                                     launch=fixed.get(name))
         bytes_per_cell = 2 * case.q * (4 if name == "f32" else 2)
         launch_ms = dev_ms / a.steps
         res[name] = {"wall_s": wall, "dev_ms": dev_ms, "launch_ms": launch_ms, "launch": lat.tuned,
@@ -320,15 +379,16 @@ def main():
         lat.close()
         for k, v in vel.items():
             err[k] = rel(v, v64)
-    copy_bytes = 4 << 30
-    copy_ms = lb.probe_copy(copy_bytes, reps=10)
-    copy_Bps = 2 * copy_bytes / (copy_ms * 1e-3)
-    probes = {}
-    for mode, name, mult in ((1, "copy4", 2), (2, "read", 1), (3, "write", 1)):
-        ms = lb.probe_bandwidth(mode, copy_bytes, reps=10)
-        probes[name] = round(mult * copy_bytes / (ms * 1e-3) / 1e9, 1)
-    probes["copy"] = round(copy_Bps / 1e9, 1)
-    log(f"[bench] HBM probes GB/s: {probes}")
+    probes = hbm_probes()
+    # the memory skeleton of this lattice's step (the same pulls and stores, no collision):
+    # the rate the access pattern itself reaches on this GPU
+    skel = {}
+    if N % 256 == 0:
+        for name in res:
+            sb = 4 if name == "f32" else 2
+            ms = lb.probe_lattice(case.q, sb, 0, case.nx, case.ny, case.nz, reps=10)
+            skel[name] = {"ms": round(ms, 4), "kernel_over_skeleton": round(res[name]["launch_ms"] / ms, 3)}
+        log(f"[bench] memory skeleton: {skel}")
     # the bluff body is quoted in its 16-bit form (BF16 deviations, 108 B per cell update)
     main_form = "bf16" if a.workload == "bluff27" and "bf16" in res else ("f32" if "f32" in res else run[0])
     r = res[main_form]
@@ -363,11 +423,13 @@ def main():
                      "traffic": traffic,
                      "bytes_per_cell_update": r["bytes_per_cell"],
                      "kernel": "step27_kernel" if case.q == 27 else "step19_kernel",
-                     "launch_ms": round(r["launch_ms"], 4), "probes_GBps": probes},
+                     "launch_ms": round(r["launch_ms"], 4), "probes_GBps": probes,
+                     "memory_skeleton": skel.get(main_form)},
         "forms": {k: {"mlups": round(v["mlups"], 1), "launch_ms": round(v["launch_ms"], 4),
                       "hbm_frac": round(v["achieved_Bps"] / HBM_PEAK, 4),
                       "velocity_rel_l2_vs_f64": (round(err[k], 8) if k in err else None),
                       "launch": v["launch"],
+                      "memory_skeleton_ms": skel.get(k, {}).get("ms"),
                       "error_steps": a.warmup + a.steps}
                   for k, v in res.items()},
     }

@@ -204,10 +204,14 @@ enum {
     LBFFS_TUNE_NO_PURE = 16,    /* every wave reads its flags before it pulls; by default a wave
                                    of 64 interior fluid cells (a per-wave summary of the flags
                                    says so) issues its pulls together with the flag load */
-    LBFFS_TUNE_FACE_FIRST = 32  /* one launch per step whose first work-groups run the face cells
+    LBFFS_TUNE_FACE_FIRST = 32, /* one launch per step whose first work-groups run the face cells
                                    (from a list) and the others the cells off the faces, so that
                                    no wave runs both paths (two-lattice streaming; takes
                                    precedence over LBFFS_TUNE_SPLIT_FACES) */
+    LBFFS_TUNE_XCD_CHUNK = 64   /* deal runs of 4 consecutive work-groups to each XCD in turn:
+                                   the cache line a block shares with its x-neighbour block is
+                                   fetched once, not once per XCD (takes precedence over
+                                   LBFFS_TUNE_NO_XCD) */
 };
 
 /* Autotuner: time every candidate work-group size (64, 128, 256) and launch option on
@@ -252,14 +256,16 @@ int lbffs_ibm_download(lbffs_ctx *ctx, int what, void *host, int n);
  * No counterpart in the reference, which runs one lattice per process (ns.cpp).  The
  * lattices share q, storage, sgs and collision; sizes, omega0 and step scalars are their
  * own.  A packed table of 256-cell blocks over all lattices drives one kernel per step;
- * the work-groups walk it with a grid-stride loop, so a grid smaller than the table gives
- * persistent work-groups.  Each lattice is an ordinary context (lbffs_batch_context) for
+ * the table is cut into 8 slabs of consecutive blocks, one per XCD, so that the blocks of a
+ * lattice share one L2; a grid smaller than the table gives persistent work-groups that
+ * walk their XCD's slab.  Each lattice is an ordinary context (lbffs_batch_context) for
  * uploads, downloads and reductions; the batch owns and destroys them, and they share the
  * batch's stream.  Moving-wall markers are not supported inside a batch. */
 typedef struct lbffs_batch lbffs_batch;
 int lbffs_batch_create(const lbffs_desc *descs, int n, lbffs_batch **out);
 int lbffs_batch_context(lbffs_batch *batch, int i, lbffs_ctx **ctx);
-/* grid: work-groups per launch (0 = one per 256-cell block); graph_chunk: steps captured
+/* grid: work-groups per launch (0 = one per 256-cell block; otherwise rounded up to a
+ * multiple of 8, at most one per block); graph_chunk: steps captured
  * into one hipGraph and replayed (0 = plain launches; the last step of a call is always a
  * plain launch) */
 int lbffs_batch_set_launch(lbffs_batch *batch, int grid, int graph_chunk);

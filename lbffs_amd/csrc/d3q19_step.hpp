@@ -87,7 +87,18 @@ enum : int {
     TUNE_SPLIT = 8,     // two passes per step: the cells off the faces, then the face cells
     TUNE_NO_PURE = 16,  // every wave waits for its flags before it pulls (no per-wave summary)
     TUNE_FACE_FIRST = 32,  // one launch: the face-list work-groups first, then the cells off the faces
+    TUNE_XCD_CHUNK = 64,   // deal runs of kXcdChunk consecutive blocks to each XCD in turn
 };
+
+// the face pass skips the face cells that the interior pass updates in their own waves:
+// those of XMIX and WRAP waves.  (Handing the x-face cells of XMIX waves to the face pass
+// instead is slower on every lattice measured: consecutive x-face cells of the list lie a
+// row apart, so each lane's pulls touch cache lines of their own, where in their own wave
+// they share the lines of their neighbours' pulls.)
+__device__ inline bool face_pass_skips(uint32_t cls)
+{
+    return cls == WAVE_WRAP || cls == WAVE_XMIX;
+}
 
 // Cell of position lin in the y-tiled order: tiles of ty full rows (ny a multiple of ty),
 // each walked plane by plane (k), row by row inside a plane.  The velocity gradient of
@@ -112,6 +123,29 @@ __device__ inline uint32_t xcd_block(uint32_t pid, uint32_t G)
     uint32_t x = pid % nx, local = pid / nx;
     uint32_t base = G / nx, r = G % nx;
     return x * base + (x < r ? x : r) + local;
+}
+
+// XCD-chunked block order: the k-th work-group of XCD x (hardware block 8k + x) takes block
+// (k / C) 8C + x C + k mod C, so each XCD runs C consecutive blocks while the eight XCDs
+// still sweep one contiguous front of the arrays.  A pull shifted by one cell in x reads
+// one cache line of its neighbour block; in hardware order that neighbour runs on another
+// XCD and the line is fetched twice (once per L2), in chunks only every C-th block edge
+// crosses XCDs.  The blocks past the last whole group of 8C keep their own number.
+constexpr uint32_t kXcdChunk = 4;
+__device__ inline uint32_t xcd_chunk_block(uint32_t g, uint32_t G)
+{
+    constexpr uint32_t span = 8 * kXcdChunk;
+    if (g >= G - G % span) return g;
+    const uint32_t k = g / 8, x = g % 8;
+    return (k / kXcdChunk) * span + x * kXcdChunk + k % kXcdChunk;
+}
+
+// block g of G in the order the launch option asks for; g % 8 must be the XCD that the
+// hardware gives block g (true when the blocks before the first are a multiple of 8)
+__device__ inline uint32_t ordered_block(uint32_t g, uint32_t G, uint32_t tune)
+{
+    if (tune & TUNE_XCD_CHUNK) return xcd_chunk_block(g, G);
+    return (tune & TUNE_NO_XCD) ? g : xcd_block(g, G);
 }
 
 // pull source for a face cell (kernel.cl:328-575 upwindID_B): wrap across non-wall faces,
@@ -733,10 +767,12 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
 // cells of the list, the others the cells off the faces, so that no wave runs both paths
 // and no second launch waits for the first to drain (the face work-groups are dispatched
 // first and run beside the interior ones).
+// (rounded up to a multiple of 8, so that the blocks after them keep their hardware XCD
+// order; the work-groups past the list end at once)
 template <typename A>
 __device__ inline uint32_t face_blocks(const A &a)
 {
-    return (a.nface + blockDim.x - 1) / blockDim.x;
+    return merged_face_blocks(a.nface, blockDim.x);
 }
 
 template <typename S, typename Real, int SGS, bool IBB = false, int AA = 0, int PASS = 0>
@@ -749,12 +785,10 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
         const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
         if (lin >= a.nface) return;
         icu = a.fcells[lin];
-        // a face cell of an XMIX or WRAP wave is updated by the interior pass
-        const uint32_t cc = cell_wave_class(a.wclass, icu);
-        if (cc == WAVE_XMIX || cc == WAVE_WRAP) return;
+        if (face_pass_skips(cell_wave_class(a.wclass, icu))) return;
     } else {
         const uint32_t g = blockIdx.x - nfb;
-        const uint32_t blk = (a.tune & TUNE_NO_XCD) ? g : xcd_block(g, gridDim.x - nfb);
+        const uint32_t blk = ordered_block(g, gridDim.x - nfb, a.tune);
         const uint32_t lin = blk * blockDim.x + threadIdx.x;
         if (lin >= a.N) return;
         icu = (a.tune & TUNE_YTILE) ? ytile_cell(a, lin) : lin;

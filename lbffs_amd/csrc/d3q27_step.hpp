@@ -646,12 +646,10 @@ step27_kernel(const StepArgs<Real, 27> a)
         const uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x;
         if (lin >= a.nface) return;
         icu = a.fcells[lin];
-        // a face cell of an XMIX or WRAP wave is updated by the interior pass
-        const uint32_t cc = cell_wave_class(a.wclass, icu);
-        if (cc == WAVE_XMIX || cc == WAVE_WRAP) return;
+        if (face_pass_skips(cell_wave_class(a.wclass, icu))) return;
     } else {
         const uint32_t g = blockIdx.x - nfb;
-        const uint32_t blk = (a.tune & TUNE_NO_XCD) ? g : xcd_block(g, gridDim.x - nfb);
+        const uint32_t blk = ordered_block(g, gridDim.x - nfb, a.tune);
         const uint32_t lin = blk * blockDim.x + threadIdx.x;
         if (lin >= a.N) return;
         icu = (a.tune & TUNE_YTILE) ? ytile_cell(a, lin) : lin;

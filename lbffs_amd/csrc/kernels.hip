@@ -12,11 +12,11 @@
 namespace lbffs {
 
 template <typename S, typename Real, int SGS, int COLL>
-void launch_step27(const StepArgs<Real, 27> &a, unsigned grid, unsigned fgrid, int block, hipStream_t st, bool merged,
-                   bool split)
+void launch_step27(const StepArgs<Real, 27> &a, unsigned grid, unsigned fgrid, unsigned mfgrid, int block,
+                   hipStream_t st, bool merged, bool split)
 {
     if (merged) {
-        hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 3, COLL>), dim3(fgrid + grid), dim3(block), 0, st, a);
+        hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 3, COLL>), dim3(mfgrid + grid), dim3(block), 0, st, a);
     } else if (split) {
         hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 1, COLL>), dim3(grid), dim3(block), 0, st, a);
         if (fgrid) hipLaunchKernelGGL((step27_kernel<S, Real, SGS, 2, COLL>), dim3(fgrid), dim3(block), 0, st, a);
@@ -31,18 +31,20 @@ void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream
     const bool merged = (a.tune & TUNE_FACE_FIRST) && aa == 0;
     const bool split = (a.tune & TUNE_SPLIT) && aa == 0 && !merged;
     const unsigned fgrid = (a.nface + (unsigned)block - 1) / (unsigned)block;
+    // the one-launch form leads with the face blocks, padded to a multiple of 8
+    const unsigned mfgrid = merged_face_blocks(a.nface, (unsigned)block);
     if constexpr (Q == 27) {
         if (a.coll == COLL_RR)
-            launch_step27<S, Real, SGS, COLL_RR>(a, grid, fgrid, block, st, merged, split);
+            launch_step27<S, Real, SGS, COLL_RR>(a, grid, fgrid, mfgrid, block, st, merged, split);
         else
-            launch_step27<S, Real, SGS, COLL_CUMULANT>(a, grid, fgrid, block, st, merged, split);
+            launch_step27<S, Real, SGS, COLL_CUMULANT>(a, grid, fgrid, mfgrid, block, st, merged, split);
     } else {
         if (aa == 1)
             hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 1>), dim3(grid), dim3(block), 0, st, a);
         else if (aa == 2)
             hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 2>), dim3(grid), dim3(block), 0, st, a);
         else if (a.ibb && merged)
-            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true, 0, 3>), dim3(fgrid + grid), dim3(block), 0, st, a);
+            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true, 0, 3>), dim3(mfgrid + grid), dim3(block), 0, st, a);
         else if (a.ibb && split) {
             hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true, 0, 1>), dim3(grid), dim3(block), 0, st, a);
             if (fgrid)
@@ -50,7 +52,7 @@ void launch_step(const StepArgs<Real, Q> &a, unsigned grid, int block, hipStream
         } else if (a.ibb) {
             hipLaunchKernelGGL((step19_kernel<S, Real, SGS, true, 0, 0>), dim3(grid), dim3(block), 0, st, a);
         } else if (merged) {
-            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 0, 3>), dim3(fgrid + grid), dim3(block), 0, st, a);
+            hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 0, 3>), dim3(mfgrid + grid), dim3(block), 0, st, a);
         } else if (split) {
             hipLaunchKernelGGL((step19_kernel<S, Real, SGS, false, 0, 1>), dim3(grid), dim3(block), 0, st, a);
             if (fgrid)
@@ -65,8 +67,8 @@ template <typename S, typename Real, int Q, int SGS, bool IBB>
 void launch_batch(const StepArgs<Real, Q> *args, const BatchBlock *table, uint32_t nblocks, const BatchBlock *ftable,
                   uint32_t nfblocks, const uint32_t *fcells, unsigned grid, int coll, hipStream_t st)
 {
-    const unsigned g = grid > 0 ? grid : nblocks;
-    const unsigned fg = grid > 0 ? (grid < nfblocks ? grid : nfblocks) : nfblocks;
+    const unsigned g = batch_grid(grid, nblocks);
+    const unsigned fg = batch_grid(grid, nfblocks);
     if constexpr (Q == 27) {
         if (coll == COLL_RR) {
             hipLaunchKernelGGL((batch_step_kernel<S, Real, SGS, Q, IBB, 1, COLL_RR>), dim3(g), dim3(kBatchBlock), 0,

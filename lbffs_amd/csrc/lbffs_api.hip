@@ -1328,15 +1328,25 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
         he = hipMemcpyAsync((char *)saved_r + k * n * rs, keep[k], n * rs, hipMemcpyDeviceToDevice, c->stream);
     if (saved_m && he == hipSuccess) he = hipMemcpyAsync(saved_m, c->markers, mbytes, hipMemcpyDeviceToDevice, c->stream);
     const int blocks[3] = {64, 128, 256};
-    // launch options: the work-group order (XCD slabs or the hardware's), the face cells in
-    // the same waves as their neighbours, in a second launch or in the first work-groups of
-    // the one launch, and the y-tiled cell order where the lattice allows it
-    int tunes[9] = {0, TUNE_NO_XCD, TUNE_FACE_FIRST, TUNE_FACE_FIRST | TUNE_NO_XCD, TUNE_SPLIT,
-                    TUNE_SPLIT | TUNE_NO_XCD, TUNE_YTILE, TUNE_FACE_FIRST | TUNE_YTILE, TUNE_SPLIT | TUNE_YTILE};
-    int ntunes = ytile_rows(c) ? 9 : 6;
+    // launch options: the work-group order (XCD slabs, XCD chunks or the hardware's), the
+    // face cells in the same waves as their neighbours, in a second launch or in the first
+    // work-groups of the one launch, and the y-tiled cell order where the lattice allows it
+    int tunes[12] = {0,
+                     TUNE_NO_XCD,
+                     TUNE_XCD_CHUNK,
+                     TUNE_FACE_FIRST,
+                     TUNE_FACE_FIRST | TUNE_NO_XCD,
+                     TUNE_FACE_FIRST | TUNE_XCD_CHUNK,
+                     TUNE_SPLIT,
+                     TUNE_SPLIT | TUNE_NO_XCD,
+                     TUNE_SPLIT | TUNE_XCD_CHUNK,
+                     TUNE_YTILE,
+                     TUNE_FACE_FIRST | TUNE_YTILE,
+                     TUNE_SPLIT | TUNE_YTILE};
+    int ntunes = ytile_rows(c) ? 12 : 9;
     if (c->d.streaming == LBFFS_STREAM_AA) {  // one pass per half-step
-        tunes[2] = TUNE_YTILE;
-        ntunes = ytile_rows(c) ? 3 : 2;
+        tunes[3] = TUNE_YTILE;
+        ntunes = ytile_rows(c) ? 4 : 3;
     }
     float bms = 1e30f;
     int bb = block0, bt = tune0;
@@ -1399,7 +1409,8 @@ int lbffs_set_launch(lbffs_ctx *c, int block, int tune)
 {
     if (!c) return fail(LBFFS_E_INVALID, "null argument");
     if (block % 64 != 0 || block < 64 || block > 256) return fail(LBFFS_E_INVALID, "block must be 64, 128, 192 or 256");
-    if (tune & ~(TUNE_NT_STORE | TUNE_NO_XCD | TUNE_YTILE | TUNE_SPLIT | TUNE_NO_PURE | TUNE_FACE_FIRST))
+    if (tune & ~(TUNE_NT_STORE | TUNE_NO_XCD | TUNE_YTILE | TUNE_SPLIT | TUNE_NO_PURE | TUNE_FACE_FIRST |
+                 TUNE_XCD_CHUNK))
         return fail(LBFFS_E_INVALID, "unknown launch option");
     c->block = block;
     c->tune = tune;

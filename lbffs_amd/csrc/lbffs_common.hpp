@@ -101,6 +101,13 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f)
 #endif
 }
 
+// work-groups of the face-cell list at the head of a one-launch step (face cells first):
+// rounded up to a multiple of 8, the number of XCDs the hardware deals blocks over in turn
+__host__ __device__ inline uint32_t merged_face_blocks(uint32_t nface, uint32_t block)
+{
+    return ((nface + block - 1) / block + 7u) & ~7u;
+}
+
 // ---- storage codecs ----
 // 16-bit forms hold (f_q - w_q) * 2^e_q; the power-of-two scale keeps the stored value
 // O(0.1..1) for every direction and is exact in both directions of the conversion.

@@ -35,7 +35,9 @@ def _run_batch(cs, nsteps, storage=lb.STORE_F32, grid=0, graph_chunk=0, chunks=1
 def test_batch_equals_single_lattices(q, storage):
     cs = cases.ensemble(6, edges=(12, 36), q=q)
     ref = [run_hip(c, 24, storage=storage) for c in cs]
-    for grid, chunk, chunks in ((0, 0, 1), (7, 0, 1), (0, 4, 2), (40, 6, 1)):
+    # grids: one work-group per block, persistent grids rounded up to a multiple of 8 (7, 13)
+    # or already one (40), with and without captured graphs
+    for grid, chunk, chunks in ((0, 0, 1), (7, 0, 1), (0, 4, 2), (40, 6, 1), (13, 3, 2)):
         out = _run_batch(cs, 24, storage=storage, grid=grid, graph_chunk=chunk, chunks=chunks)
         for d, (o, r) in enumerate(zip(out, ref)):
             for k in KEYS:

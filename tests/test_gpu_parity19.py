@@ -111,12 +111,25 @@ def test_block_size_invariance(block):
     assert np.array_equal(ref["f"], h["f"])
 
 
+@pytest.mark.parametrize("tune", [64, 96])
+def test_xcd_chunk_order_ragged_tail(tune):
+    # 8000 cells in blocks of 64: 125 blocks, three whole groups of 8 x 4 and a tail of 29
+    # that keeps the hardware order; every cell still runs once
+    c = cases.cavity(20)
+    ref = run_hip(c, 30)
+    h = run_hip(c, 30, block=64, tune=tune)
+    assert np.array_equal(ref["f"], h["f"])
+    assert np.array_equal(ref["u"], h["u"])
+
+
 @pytest.mark.parametrize("sgs", [lb.SGS_SMAGORINSKY, lb.SGS_WALE])
-@pytest.mark.parametrize("tune", [1, 2, 3, 4, 6, 8, 10, 12, 16, 32, 34, 36, 48])
+@pytest.mark.parametrize("tune", [1, 2, 3, 4, 6, 8, 10, 12, 16, 32, 34, 36, 48, 64, 66, 68, 72, 96])
 def test_launch_options_bitwise_identical(tune, sgs):
     # non-temporal stores, the work-group order, the y-tiled cell order (4: tiles of 16
-    # rows here), the split face pass (8), the per-wave flag summary switched off (16) and
-    # the face cells in the first work-groups of the one launch (32) change no arithmetic
+    # rows here), the split face pass (8), the per-wave flag summary switched off (16),
+    # the face cells in the first work-groups of the one launch (32) and the XCD-chunked
+    # order (64: the 32 blocks of this lattice are one whole group of 8 x 4) change no
+    # arithmetic
     c = cases.cylinder_channel(nx=64, ny=32, nz=4, D=8, cx=20, cy=16, spz_cells=6, sgs=sgs)
     a = run_hip(c, 25)
     b = run_hip(c, 25, tune=tune)
@@ -124,7 +137,7 @@ def test_launch_options_bitwise_identical(tune, sgs):
         assert np.array_equal(a[k], b[k]), k
 
 
-@pytest.mark.parametrize("tune", [8, 32, 34])
+@pytest.mark.parametrize("tune", [8, 32, 34, 96])
 def test_launch_options_bitwise_identical_ibb(tune):
     # the interpolated bounce-back variant of the kernel under the face-pass launch options
     c = cases.cylinder_channel(nx=64, ny=32, nz=4, D=8, cx=20, cy=16, spz_cells=6, ibb=True)
@@ -171,7 +184,7 @@ def test_reference_kernel_cavity_short():
     e_rh = compare(h, r, keys=("rho", "vel", "f"), tol=1e-3, label="hip vs refcl")
 
 
-@pytest.mark.parametrize("tune", [8, 10, 12, 16, 32, 34])
+@pytest.mark.parametrize("tune", [8, 10, 12, 16, 32, 34, 64, 96])
 def test_launch_options_bitwise_identical_16bit(tune):
     # stochastic rounding draws its bits from the cell's own state: the split face pass and
     # the y-tiled order leave the FP16 results bit-identical too

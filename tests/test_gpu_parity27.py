@@ -94,7 +94,7 @@ def test_store_modes_bitwise_identical27():
         assert np.array_equal(a[k], d[k]), k
 
 
-@pytest.mark.parametrize("tune", [1, 2, 3, 4, 6, 8, 10, 12, 16, 32, 34, 36])
+@pytest.mark.parametrize("tune", [1, 2, 3, 4, 6, 8, 10, 12, 16, 32, 34, 36, 64, 96])
 def test_launch_options_bitwise_identical27(tune):
     c = cases.cavity(16, q=27)
     a = run_hip(c, 20)
@@ -242,7 +242,7 @@ def test_cylinder27_rr_vs_oracle(ibb):
     assert rel_l2(fw_h, fw_o) <= 1e-4, (fw_h, fw_o)
 
 
-@pytest.mark.parametrize("tune", [8, 32])
+@pytest.mark.parametrize("tune", [8, 32, 96])
 def test_rr27_launch_options_bitwise_identical(tune):
     c = cases.cavity(16, q=27)
     c.collision = lb.COLL_RR
