@@ -50,7 +50,7 @@ enum {
 /* ---- sub-grid-scale models ---- */
 enum {
     LBFFS_SGS_WALE = 0,        /* waleTauSGS, the model the reference kernel runs (K19:3631) */
-    LBFFS_SGS_SMAGORINSKY = 1, /* smagorinskyTauSGS with Fluent-style wall damping (K19:2133-2201) */
+    LBFFS_SGS_SMAGORINSKY = 1, /* smagorinskyTauSGS with wall damping (K19:2133-2201; see damping) */
     LBFFS_SGS_NONE = 2         /* tauSGS = 0 */
 };
 
@@ -134,7 +134,10 @@ typedef struct {
     float omegaB;    /* bulk relaxation rate (RR collision; omega2 of the cumulant collision) */
     float spzWidth;  /* sponge width as a fraction of the domain (tauSpongeZone) */
     float Cs;        /* Smagorinsky constant */
-    int damping;     /* Smagorinsky wall damping on (1) / off (0) */
+    int damping;     /* Smagorinsky wall damping at the distance sdf: 0 off; 1 the Fluent-style
+                        min(1, 0.41 y / Cs)^2 the reference applies (K19:2170-2177); 2 van
+                        Driest, the branch K19:2178-2197 keeps in comments (y+ from the log
+                        law by Newton's method, A+ = 26) */
     int store_every; /* 1: write rho/u/v/w/tauSGS/Fw/G for every cell on every step
                         (reference semantics throughout); 0: only on the last step of
                         the call and for the cells later steps read */

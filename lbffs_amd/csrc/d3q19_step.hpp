@@ -465,6 +465,28 @@ __device__ inline Mom19<Real> moments19_dev(const Real *ft)
     return m;
 }
 
+// van Driest wall damping of tauSGS (damping 2), the branch kernel.cl:2178-2197 keeps in
+// comments beside the Fluent factor: y+ solves the log law y+ ln(E y+) = kappa U y / nu by
+// Newton's method from y+ = 1 (stop when a step moves y+ by 1e-3 or less, or after 10
+// steps; from y+ = 1 every iterate stays above 1/E, where 1 + ln(E y+) > 0), and the factor
+// is min(1, (kappa / Cs)(1 + 1e-10 - exp(-y+ / A+)) y)^2, A+ = 26, E = 9.8,
+// nu = (tau - 1/2) / 3.
+template <typename Real>
+__device__ inline Real van_driest(Real tau, Real U, Real y, Real Cs)
+{
+    const Real kappa = Real(0.41), Aplus = Real(26), E = Real(9.8);
+    const Real nu = (tau - Real(0.5)) / Real(3);
+    Real yp = Real(1), ypn;
+    int it = 0;
+    do {
+        ypn = yp;
+        yp = ((kappa * U * y / nu) + yp) / (Real(1) + log(E * yp));
+        it++;
+    } while (fabs(yp - ypn) > Real(0.001) && it < 10);
+    const Real d = fmin(Real(1), (kappa / Cs) * ((Real(1) + Real(1e-10)) - exp(-yp / Aplus)) * y);
+    return d * d;
+}
+
 // kernel.cl:2133-2201 smagorinskyTauSGS.  The reference sums c_a c_b (f_q - feq_q) over q
 // with the incompressible feq (rho + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2) w_q, whose second
 // moment is rho/3 delta_ab + u_a u_b; Pi_neq is formed from the shared moments instead.
@@ -481,6 +503,8 @@ __device__ inline Real smagorinsky_tau(const Mom19<Real> &m, Real u, Real v, Rea
     if (damping == 1 && sdf != Real(10000) && sdf > Real(0)) {
         const Real d = fmin(Real(1), Real(0.41) * sdf / Cs);
         t *= d * d;
+    } else if (damping == 2 && sdf != Real(10000) && sdf > Real(0)) {
+        t *= van_driest<Real>(tau, sqrt(u * u + v * v + w * w), sdf, Cs);
     }
     return t;
 }

@@ -1264,10 +1264,18 @@ int lbffs_upload(lbffs_ctx *c, int field, const void *host, size_t bytes)
     return 0;
 }
 
+// step scalars that select code paths
+static int params_ok(const lbffs_step_params *p)
+{
+    if (p->damping < 0 || p->damping > 2) return fail(LBFFS_E_INVALID, "damping must be 0, 1 or 2");
+    return 0;
+}
+
 int lbffs_step(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
 {
     if (!c || !p) return fail(LBFFS_E_INVALID, "null argument");
     if (nsteps < 0) return fail(LBFFS_E_INVALID, "nsteps < 0");
+    if (int e = params_ok(p)) return e;
     if (!c->f_loaded) return fail(LBFFS_E_STATE, "populations were never uploaded");
     if (p->ibb && !c->qlink) return fail(LBFFS_E_STATE, "ibb = 1 needs the q-distances (LBFFS_FIELD_QLINK)");
     if (int e = aa_step_ok(c, p)) return e;
@@ -1290,6 +1298,7 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
 {
     if (!c || !p) return fail(LBFFS_E_INVALID, "null argument");
     if (trial_steps < 1) return fail(LBFFS_E_INVALID, "trial_steps < 1");
+    if (int e = params_ok(p)) return e;
     if (!c->f_loaded) return fail(LBFFS_E_STATE, "populations were never uploaded");
     if (p->ibb && !c->qlink) return fail(LBFFS_E_STATE, "ibb = 1 needs the q-distances (LBFFS_FIELD_QLINK)");
     if (int e = aa_step_ok(c, p)) return e;
@@ -1655,6 +1664,7 @@ int lbffs_batch_step(lbffs_batch *b, int nsteps, const lbffs_step_params *params
         if (ibm_active(c)) return fail(LBFFS_E_UNSUPPORTED, "moving-wall markers in a batch");
         if (params[d].ibb && !c->qlink)
             return fail(LBFFS_E_STATE, "lattice " + std::to_string(d) + ": ibb = 1 needs the q-distances");
+        if (int e = params_ok(&params[d])) return e;
         if (c->cur != cur0 || c->mcur != cur0)
             return fail(LBFFS_E_STATE, "lattice " + std::to_string(d) + " was stepped on its own: step parity differs");
         if (c->flags_dirty) {

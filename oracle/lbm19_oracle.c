@@ -178,6 +178,7 @@ int orc_corner(int b3, int i, int j, int k, int nx, int ny, int nz)
 #define SQRT sqrtf
 #define POW powf
 #define EXP expf
+#define LOG logf
 #define FABS fabsf
 #define FMIN fminf
 #define FMAX fmaxf
@@ -187,6 +188,7 @@ int orc_corner(int b3, int i, int j, int k, int nx, int ny, int nz)
 #undef SQRT
 #undef POW
 #undef EXP
+#undef LOG
 #undef FABS
 #undef FMIN
 #undef FMAX
@@ -197,6 +199,7 @@ int orc_corner(int b3, int i, int j, int k, int nx, int ny, int nz)
 #define SQRT sqrt
 #define POW pow
 #define EXP exp
+#define LOG log
 #define FABS fabs
 #define FMIN fmin
 #define FMAX fmax
@@ -206,6 +209,7 @@ int orc_corner(int b3, int i, int j, int k, int nx, int ny, int nz)
 #undef SQRT
 #undef POW
 #undef EXP
+#undef LOG
 #undef FABS
 #undef FMIN
 #undef FMAX

@@ -23,7 +23,7 @@ typedef struct {
     int order;     /* ORC_JACOBI | ORC_LEX_GS */
     int sgs;       /* ORC_SGS_* */
     int collision; /* ORC_COLL_* */
-    int damping;   /* Smagorinsky wall damping on/off */
+    int damping;   /* Smagorinsky wall damping: 0 off, 1 Fluent, 2 van Driest */
     float dpdx, LES, omegaB, spzWidth, Cs;
     float clim;    /* D3Q27 cumulant limiter (input.hpp default 0.01) */
     int ibb;       /* 1: interpolated bounce-back with the precomputed q-distances below */
@@ -59,6 +59,9 @@ float orc19_smagorinsky_f32(const float *ft, float rho, float u, float v, float 
                             float Cs, int damping);
 double orc19_smagorinsky_f64(const double *ft, double rho, double u, double v, double w, double omega, double LES,
                              double sdf, double Cs, int damping);
+/* van Driest damping factor of tauSGS (damping 2) at wall distance y and speed U; y+ out */
+float orc_van_driest_f32(float tau, float U, float y, float Cs, float *yplus);
+double orc_van_driest_f64(double tau, double U, double y, double Cs, double *yplus);
 
 /* D3Q27 */
 extern const int ORC27_CX[27], ORC27_CY[27], ORC27_CZ[27];

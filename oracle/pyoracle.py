@@ -71,6 +71,10 @@ def _bind(L):
     L.orc19_smagorinsky_f32.restype = C.c_float
     L.orc19_smagorinsky_f64.argtypes = [vp] + [C.c_double] * 8 + [C.c_int]
     L.orc19_smagorinsky_f64.restype = C.c_double
+    L.orc_van_driest_f32.argtypes = [C.c_float] * 4 + [vp]
+    L.orc_van_driest_f32.restype = C.c_float
+    L.orc_van_driest_f64.argtypes = [C.c_double] * 4 + [vp]
+    L.orc_van_driest_f64.restype = C.c_double
     L.orc_upwind_B.argtypes = [C.c_int] * 10
     L.orc_upwind_B.restype = C.c_long
     L.orc_reflect_or_mirror.argtypes = [C.c_int] * 4
@@ -258,6 +262,13 @@ def smagorinsky(ft, rho, u, v, w, omega, LES=1.0, sdf=10000.0, Cs=0.1, damping=1
     real = np.float64 if precision == "f64" else np.float32
     a = np.ascontiguousarray(ft, real)
     return getattr(orc(), f"orc19_smagorinsky_{precision}")(_p(a), rho, u, v, w, omega, LES, sdf, Cs, damping)
+
+
+def van_driest(tau, U, y, Cs=0.1, precision="f64"):
+    """(damping factor, y+) of the van Driest wall damping (damping = 2)."""
+    yp = (C.c_double if precision == "f64" else C.c_float)()
+    f = getattr(orc(), f"orc_van_driest_{precision}")(tau, U, y, Cs, C.byref(yp))
+    return f, yp.value
 
 
 # ---- the reference kernel, run on the GPU ----

@@ -47,6 +47,27 @@ def test_cylinder_channel_vs_oracle():
     assert abs(h["reduce"][0] - fw_h[0]) <= 1e-6 * max(1.0, abs(fw_h[0]))
 
 
+@pytest.mark.parametrize("damping", [1, 2])
+def test_smagorinsky_wall_damping_vs_oracle(damping):
+    # Smagorinsky LES with the wall distance of the cylinder's near cells: the Fluent
+    # factor (1) and van Driest (2, y+ from the log law by Newton's method)
+    c = cases.cylinder_channel(nx=96, ny=48, nz=8, D=12, cx=30, cy=24, Re=60.0, spz_cells=8,
+                               sgs=lb.SGS_SMAGORINSKY, sdf_from_geometry=True)
+    c.params.update(LES=1.0, damping=damping)
+    h, o, _ = parity(c, 150, label=f"cylinder smagorinsky damping={damping}")
+    near = c.arrays["sdf"] != 10000.0
+    assert near.sum() > 100
+    if damping == 2:
+        # van Driest damps less than nothing and more than the Fluent factor
+        c.params.update(damping=1)
+        f1 = run_hip(c, 150)
+        c.params.update(damping=0)
+        f0 = run_hip(c, 150)
+        t2, t1, t0 = (np.asarray(x["tauSGS"], np.float64)[near] for x in (h, f1, f0))
+        assert not np.array_equal(t2, t1)
+        assert np.abs(t2).sum() < np.abs(t0).sum()
+
+
 @pytest.mark.parametrize("sgs", [lb.SGS_WALE, lb.SGS_SMAGORINSKY])
 def test_cylinder_channel_ibb_vs_oracle(sgs):
     # Filippova-Hanel interpolated bounce-back with analytic q-distances (ibb = 1)
@@ -58,6 +79,14 @@ def test_cylinder_channel_ibb_vs_oracle(sgs):
     # the interpolated walls move the drag away from the bounce-back run's
     b = run_hip(cases.cylinder_channel(nx=96, ny=48, nz=8, D=12, cx=30, cy=24, Re=60.0, spz_cells=8, sgs=sgs), 150)
     assert abs(b["Fwx"].astype(np.float64).sum() - fw_h[0]) > 1e-3 * abs(fw_h[0])
+
+
+def test_unknown_damping_mode_is_refused():
+    c = cases.cavity(8, sgs=lb.SGS_SMAGORINSKY)
+    with lb.Lattice(c.nx, c.ny, c.nz, sgs=c.sgs, omega0=c.omega0) as lat:
+        c.upload_to(lat)
+        with pytest.raises(lb.LbffsError, match="damping"):
+            lat.step(1, **{**c.params, "damping": 3})
 
 
 def test_ibb_without_qlinks_is_refused():

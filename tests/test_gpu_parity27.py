@@ -51,6 +51,14 @@ def test_cylinder27_ibb_vs_oracle(sdf_from_geometry):
     assert rel_l2(fw_h, fw_o) <= 1e-4, (fw_h, fw_o)
 
 
+def test_cylinder27_van_driest_vs_oracle():
+    # Smagorinsky LES with van Driest wall damping near the cylinder (damping = 2)
+    c = cases.cylinder_channel(nx=64, ny=32, nz=6, D=10, cx=20, cy=16, Re=60.0, spz_cells=6, q=27,
+                               sgs=lb.SGS_SMAGORINSKY, sdf_from_geometry=True)
+    c.params.update(LES=1.0, damping=2)
+    parity(c, 100, label="cylinder q27 smagorinsky van Driest")
+
+
 def test_cylinder27_qlinks_vs_oracle():
     # interpolated bounce-back with analytic q-distances instead of the distance ratio
     c = cases.cylinder_channel(nx=64, ny=32, nz=6, D=10, cx=20, cy=16, Re=60.0, spz_cells=6, q=27, ibb=True)

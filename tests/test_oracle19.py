@@ -166,6 +166,55 @@ def test_smagorinsky_closed_form(damping, sdf):
     assert abs(got - exp) < 1e-12 * max(1.0, abs(exp)) + 1e-15
 
 
+@pytest.mark.parametrize("tau,U,y", [(0.51, 0.05, 1.5), (0.6, 0.1, 0.5), (0.5003, 0.08, 3.0), (0.9, 0.0, 2.0),
+                                     (0.5001, 0.2, 10.0)])
+def test_van_driest_solves_the_log_law(tau, U, y):
+    """van Driest damping (the branch kernel.cl:2178-2197 keeps in comments): its y+ is the
+    root of the log law y+ ln(E y+) = kappa U y / nu (E = 9.8, nu = (tau - 1/2)/3), found
+    here by a bracketing root finder; the factor never exceeds the Fluent factor
+    min(1, 0.41 y / Cs)^2 it refines."""
+    from scipy.optimize import brentq
+    Cs = 0.1
+    f, yp = po.van_driest(tau, U, y, Cs)
+    R = 0.41 * U * y / ((tau - 0.5) / 3)
+    exact = brentq(lambda t: t * np.log(9.8 * t) - R, 1 / 9.8, max(10.0, 10 * R))
+    assert abs(yp - exact) <= 1e-3 * max(1.0, exact), (yp, exact)
+    fluent = min(1.0, 0.41 * y / Cs) ** 2
+    assert 0.0 <= f <= fluent
+    # FP32 may stop the iteration one step apart (|dy+| <= 1e-3), which moves the factor
+    # (1 - exp(-y+/26))^2 by at most 2e-3 / y+ relative
+    f32, yp32 = po.van_driest(tau, U, y, Cs, precision="f32")
+    assert abs(f32 - f) <= (2e-3 / min(yp, 26.0) + 1e-5) * f
+
+
+def test_van_driest_reduces_to_the_fluent_factor_far_from_the_wall():
+    # y+ >> A+ = 26: exp(-y+/26) vanishes and the factor is the reference's active one
+    tau, U, Cs = 0.5 + 1e-6, 0.2, 0.1
+    for y in (0.05, 0.1, 0.2, 0.3):
+        f, yp = po.van_driest(tau, U, y, Cs)
+        assert yp > 26 * 30
+        # (the factor's 1 + 1e-10, as the reference writes it, makes it 2e-10 larger)
+        fl = min(1.0, 0.41 * y / Cs) ** 2
+        assert abs(f - fl) <= 3e-10 * fl
+
+
+def test_smagorinsky_van_driest_damps_the_undamped_value():
+    rng = np.random.default_rng(5)
+    rho, u = 1.01, np.array([0.04, -0.02, 0.01])
+    f = feq(rho, *u) * (1 + 1e-3 * rng.standard_normal(19))
+    omega, Cs = 1 / 0.53, 0.1
+    for sdf in (0.5, 1.5, 4.0):
+        free = po.smagorinsky(f, rho, *u, omega, 1.0, sdf, Cs, 0)
+        damped = po.smagorinsky(f, rho, *u, omega, 1.0, sdf, Cs, 2)
+        fac, _ = po.van_driest(1 / omega, float(np.linalg.norm(u)), sdf, Cs)
+        assert free > 0
+        assert abs(damped - free * fac) <= 1e-14 * free
+        assert damped <= po.smagorinsky(f, rho, *u, omega, 1.0, sdf, Cs, 1) + 1e-18
+    # off the walls (sdf = 10000) neither damping applies
+    assert po.smagorinsky(f, rho, *u, omega, 1.0, 10000.0, Cs, 2) == po.smagorinsky(f, rho, *u, omega, 1.0, 10000.0,
+                                                                                       Cs, 0)
+
+
 # ---------------------------------------------------------------- whole-step invariants
 
 def _box_state(case, rng, amp=1e-3, vel=(0.0, 0.0, 0.0)):
