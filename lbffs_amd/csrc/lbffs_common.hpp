@@ -101,6 +101,21 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f)
 #endif
 }
 
+// p held as an opaque wave-uniform value in scalar registers.  Stores of one lane offset
+// into several arrays (new populations: direction q at fd + q FS + offset) would otherwise
+// be rewritten by the compiler into one 64-bit vector address advanced by a 64-bit vector
+// add per store; through this barrier each store takes its array's scalar base and the
+// shared 32-bit lane offset.  (An empty asm statement: no instruction is emitted.)
+template <typename T>
+__device__ inline T *scalar_base(T *p)
+{
+    uint64_t a = (uint64_t)p;
+    asm("" : "+s"(a));
+    // back as a global-memory pointer (a plain integer cast would make the stores flat ones)
+    typedef __attribute__((address_space(1))) T gT;
+    return (T *)(gT *)a;
+}
+
 // work-groups of the face-cell list at the head of a one-launch step (face cells first):
 // rounded up to a multiple of 8, the number of XCDs the hardware deals blocks over in turn
 __host__ __device__ inline uint32_t merged_face_blocks(uint32_t nface, uint32_t block)
@@ -267,7 +282,9 @@ struct Codec<__half, float, Q> {
                                                float w1, uint32_t r0, uint32_t r1)
     {
         typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
-        h2_t h = {0, 0};
+        // both halves are written below: start from the register of the random bits, which
+        // the first conversion consumes, rather than from a cleared one
+        h2_t h = __builtin_bit_cast(h2_t, r0);
         h = __builtin_amdgcn_cvt_sr_f16_f32(h, dev_scaled<Q>(v0, w0, q0), r0, false);
         h = __builtin_amdgcn_cvt_sr_f16_f32(h, dev_scaled<Q>(v1, w1, q1), r1, true);
         const _Float16 e0 = h[0], e1 = h[1];
@@ -308,7 +325,7 @@ struct Codec<bf16_t, float, Q> {
                                                float w1, uint32_t r0, uint32_t r1)
     {
         typedef short b2_t __attribute__((ext_vector_type(2)));
-        b2_t h = {0, 0};
+        b2_t h = __builtin_bit_cast(b2_t, r0);  // both halves are written below (as for FP16)
         h = __builtin_amdgcn_cvt_sr_bf16_f32(h, dev_scaled<Q>(v0, w0, q0), r0, false);
         h = __builtin_amdgcn_cvt_sr_bf16_f32(h, dev_scaled<Q>(v1, w1, q1), r1, true);
         __builtin_nontemporal_store((unsigned short)h[0], (unsigned short *)p0);
