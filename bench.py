@@ -35,6 +35,8 @@ from lbffs_amd import cases  # noqa: E402
 HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
 PROFILE_PMC = os.path.join(ROOT, "profiles", "step19_backstep_pmc.txt")
 PROFILE_ENS_PMC = os.path.join(ROOT, "profiles", "batch_ensemble_pmc.txt")
+PROFILE_BLUFF_PMC = os.path.join(ROOT, "profiles", "step27_bluff_pmc.txt")
+PMC_OF_WORKLOAD = {"backstep": PROFILE_PMC, "bluff27": PROFILE_BLUFF_PMC}
 # bytes one ensemble cell update must move: 19 FP16 populations in and out, and the
 # WALE model's FP32 velocity (its gradient reads the neighbours' velocities of the previous
 # step) written once and read once
@@ -361,7 +363,9 @@ def main():
         lat, wall, dev_ms = run_form(case, forms[name], a.warmup, a.steps, rounding=ROUNDING[a.rounding],
                                      streaming=STREAMING[a.streaming], trials=a.tune_trials,
                                      launch=fixed.get(name))
-        bytes_per_cell = 2 * case.q * (4 if name == "f32" else 2)
+        # populations in and out, and with WALE the FP32 velocity its gradient reads from the
+        # previous step: written once and read once per cell update
+        bytes_per_cell = 2 * case.q * (4 if name == "f32" else 2) + (24 if case.sgs == lb.SGS_WALE else 0)
         launch_ms = dev_ms / a.steps
         res[name] = {"wall_s": wall, "dev_ms": dev_ms, "launch_ms": launch_ms, "launch": lat.tuned,
                      "mlups": a.steps * N / wall / 1e6, "bytes_per_cell": bytes_per_cell,
@@ -389,10 +393,11 @@ def main():
             ms = lb.probe_lattice(case.q, sb, 0, case.nx, case.ny, case.nz, reps=10)
             skel[name] = {"ms": round(ms, 4), "kernel_over_skeleton": round(res[name]["launch_ms"] / ms, 3)}
         log(f"[bench] memory skeleton: {skel}")
-    # the bluff body is quoted in its 16-bit form (BF16 deviations, 108 B per cell update)
+    # the bluff body is quoted in its 16-bit form (BF16 deviations, 108 + 24 B per cell update)
     main_form = "bf16" if a.workload == "bluff27" and "bf16" in res else ("f32" if "f32" in res else run[0])
     r = res[main_form]
-    traffic = read_pmc() if a.workload == "backstep" and main_form == "f32" else None
+    pmc = PMC_OF_WORKLOAD.get(a.workload)
+    traffic = read_pmc(pmc, "traffic_bytes_per_launch" + ("" if main_form == "f32" else "_" + main_form)) if pmc else None
     out = {
         "metric": "MLUPS + % of HBM roofline, FP32 vs FP16-storage D3Q19 at stated error",
         "value": round(r["mlups"], 1),
@@ -429,6 +434,8 @@ def main():
                       "hbm_frac": round(v["achieved_Bps"] / HBM_PEAK, 4),
                       "velocity_rel_l2_vs_f64": (round(err[k], 8) if k in err else None),
                       "launch": v["launch"],
+                      "traffic": (read_pmc(pmc, "traffic_bytes_per_launch" + ("" if k == "f32" else "_" + k))
+                                  if pmc else None),
                       "memory_skeleton_ms": skel.get(k, {}).get("ms"),
                       "error_steps": a.warmup + a.steps}
                   for k, v in res.items()},
