@@ -64,36 +64,42 @@ def test_c1_cavity64_2000_steps_vs_oracle():
 
 def test_c1_cavity64_2000_steps_reference_kernel_within_race_bracket():
     """The reference kernel.cl (compiled here for gfx950, run with its own host loop) at
-    three work-group sizes, and the oracle in its two neighbour-read orders.  The read
-    order is the reference's race: the bracket is the largest distance among these runs.
-    The reference runs lie within the bracket of the Jacobi oracle and of the HIP step
-    (plus the FP32 rounding level), and the bracket stays below 1e-3."""
+    three work-group sizes, and the oracle in its two neighbour-read orders.  The read order
+    is the reference's race, so its effect is bracketed twice: by the spread of the reference
+    runs across work-group sizes and by the distance between the oracle's Jacobi and
+    lexicographic orders.  The reference's distance from the oracle beyond that bracket is
+    the same at every work-group size -- its fast-math arithmetic (ns.cpp:398), not the race.
+    Asserted: the race bracket is below 1e-5; the reference lies within 1e-4 of the oracle;
+    the HIP step is no further from the reference than the literal FP32 oracle is (plus the
+    FP32 rounding level)."""
     if "h32" not in _C1:
         test_c1_cavity64_2000_steps_vs_oracle()
     c, h32, o32, fl = _C1["case"], _C1["h32"], _C1["o32"], _C1["floor"]
     n = 2000
     refs = {wg: po.run_reference_gpu(c, n, block=wg) for wg in (64, 128, 256)}
     lex = po.run19(c, n, order=po.LEX_WAVE)
-    runs = {"jacobi": o32, "lexicographic": lex, **{f"ref wg{wg}": r for wg, r in refs.items()}}
-    names = list(runs)
-    spread = {}
-    for k in ("rho", "vel", "f"):
-        spread[k] = max(_errs(runs[a], runs[b], (k,))[k] for i, a in enumerate(names) for b in names[i + 1:])
-    rec = {"bracket": spread, "floor": fl,
-           "orders": _errs(o32, lex, ("rho", "vel", "f")),
-           "refs": {wg: {"ref-o32": _errs(r, o32, ("rho", "vel", "f")), "hip-ref": _errs(h32, r, ("rho", "vel", "f"))}
+    keys = ("rho", "vel", "f")
+    wgs = list(refs)
+    ref_spread = {k: max(_errs(refs[a], refs[b], (k,))[k] for i, a in enumerate(wgs) for b in wgs[i + 1:])
+                  for k in keys}
+    orders = _errs(o32, lex, keys)
+    race = {k: max(ref_spread[k], orders[k]) for k in keys}
+    rec = {"race_bracket": race, "ref_spread_over_wg": ref_spread, "oracle_orders": orders, "floor": fl,
+           "hip_o32": _errs(h32, o32, keys),
+           "refs": {wg: {"ref-o32": _errs(r, o32, keys), "ref-lex": _errs(r, lex, keys), "hip-ref": _errs(h32, r, keys)}
                     for wg, r in refs.items()}}
-    print("C1 race bracket", {k: f"{v:.2e}" for k, v in spread.items()}, "jacobi-lex",
-          {k: f"{v:.2e}" for k, v in rec["orders"].items()}, flush=True)
+    print("C1 race bracket", {k: f"{v:.2e}" for k, v in race.items()}, "ref spread over wg",
+          {k: f"{v:.2e}" for k, v in ref_spread.items()}, "jacobi-lex", {k: f"{v:.2e}" for k, v in orders.items()},
+          flush=True)
     for wg, d in rec["refs"].items():
         print(f"  wg {wg}: ref-o32", {k: f"{v:.2e}" for k, v in d["ref-o32"].items()}, "hip-ref",
               {k: f"{v:.2e}" for k, v in d["hip-ref"].items()}, flush=True)
     _log("C1 cavity64 x2000 race bracket", rec)
-    for k in ("rho", "vel", "f"):
-        assert spread[k] < 1e-3, (k, spread[k])
+    for k in keys:
+        assert race[k] < 1e-5, (k, race[k])
         for wg, d in rec["refs"].items():
-            assert d["ref-o32"][k] <= spread[k] + 2 * fl[k] + 1e-7, (wg, k)
-            assert d["hip-ref"][k] <= spread[k] + 2 * fl[k] + 1e-7, (wg, k)
+            assert d["ref-o32"][k] < 1e-4, (wg, k, d["ref-o32"][k])
+            assert d["hip-ref"][k] <= d["ref-o32"][k] + 2 * fl[k] + 1e-7, (wg, k)
 
 
 # ---------------------------------------------------------------- C3: FP16 mean profiles
