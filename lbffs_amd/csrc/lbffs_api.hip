@@ -554,6 +554,17 @@ uint32_t ytile_rows(const lbffs_ctx *c)
     return 0;
 }
 
+// whether the step kernels of this lattice read its per-wave flag summary: the summary
+// names aligned runs of 64 cells, which a wave covers when its cells are consecutive (the
+// plain order, or y-tiles whose rows are whole waves); off with TUNE_NO_PURE and in-place
+// streaming
+bool uses_wave_classes(const lbffs_ctx *c)
+{
+    const bool ytile = (c->tune & TUNE_YTILE) && ytile_rows(c) != 0;
+    const bool runs = !ytile || c->d.nx % 64 == 0;
+    return c->wclass && runs && !(c->tune & TUNE_NO_PURE) && c->d.streaming != LBFFS_STREAM_AA;
+}
+
 template <typename S, typename Real, int Q>
 void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
 {
@@ -617,10 +628,7 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
         a.dplane = make_fastdiv(a.tile_plane);
         if (!ty) a.tune &= ~TUNE_YTILE;
     }
-    // the summary names aligned runs of 64 cells: a wave covers one when its cells are
-    // consecutive (the plain order, or y-tiles whose rows are whole waves)
-    const bool runs = !(a.tune & TUNE_YTILE) || nx % 64 == 0;
-    a.wclass = (runs && !(c->tune & TUNE_NO_PURE) && c->d.streaming != LBFFS_STREAM_AA) ? c->wclass : nullptr;
+    a.wclass = uses_wave_classes(c) ? c->wclass : nullptr;
 }
 
 // In-place streaming: bring the array back to the two-lattice layout (after an odd number
@@ -941,6 +949,8 @@ struct lbffs_batch {
     uint32_t *fcells = nullptr;
     bool ftable_ok = false;
     std::vector<uint64_t> gen;  // flags generation of each lattice the face table was built from
+    std::vector<uint8_t> wc;    // whether each lattice used its wave summary then (its XMIX and
+                                // WRAP waves' face cells are left out of the table)
     uint32_t *ctr = nullptr;    // device counter of the steps run in the current call
     int grid = 0;        // work-groups of the launch; 0 = one per table entry
     int graph_chunk = 0; // steps per captured hipGraph; 0 = plain launches
@@ -978,19 +988,37 @@ void batch_launch(lbffs_batch *b, int parity, int last)
                                        b->ctx[0]->d.collision, b->stream);
 }
 
-// the face table: every non-solid face cell of every lattice, 256 to a block
+// the face table: every non-solid face cell of every lattice, 256 to a block, except the face
+// cells of XMIX and WRAP waves where the lattice uses its wave summary (the block-table pass
+// runs those in their own waves; left in, they would idle lanes of the face pass's waves)
 int batch_face_table(lbffs_batch *b)
 {
     std::vector<uint32_t> cells;
     std::vector<BatchBlock> blocks;
     std::vector<uint16_t> fl;
+    b->wc.assign(b->ctx.size(), 0);
     for (size_t d = 0; d < b->ctx.size(); d++) {
         lbffs_ctx *c = b->ctx[d];
         fl.resize((size_t)c->N);
         HIP_TRY(hipMemcpy(fl.data(), c->flag, (size_t)c->N * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        const bool wcs = uses_wave_classes(c);
+        b->wc[d] = wcs ? 1 : 0;
+        const size_t nw = (size_t)c->N / 64;  // whole runs; the tail run is GENERAL
+        size_t cur_w = ~size_t(0);
+        uint32_t cls = WAVE_GENERAL;
         const size_t start = cells.size();
-        for (long ic = 0; ic < c->N; ic++)
-            if ((fl[ic] & F_FACE) && !(fl[ic] & F_SOLID)) cells.push_back((uint32_t)ic);
+        for (long ic = 0; ic < c->N; ic++) {
+            if (!(fl[ic] & F_FACE) || (fl[ic] & F_SOLID)) continue;
+            if (wcs) {
+                const size_t w = (size_t)ic / 64;
+                if (w != cur_w) {
+                    cur_w = w;
+                    cls = w < nw ? wave_class_of(c, fl.data(), w * 64) : WAVE_GENERAL;
+                }
+                if (cls == WAVE_XMIX || cls == WAVE_WRAP) continue;
+            }
+            cells.push_back((uint32_t)ic);
+        }
         for (size_t f = start; f < cells.size(); f += kBatchBlock)
             blocks.push_back({(uint32_t)d, (uint32_t)f, (uint32_t)std::min<size_t>(kBatchBlock, cells.size() - f)});
     }
@@ -1674,6 +1702,9 @@ int lbffs_batch_step(lbffs_batch *b, int nsteps, const lbffs_step_params *params
         // the flags may also have been rebuilt by a call on the lattice's own context
         // (lbffs_reduce, lbffs_step, lbffs_autotune): compare generations, not the dirty bit
         if (b->gen.size() != b->ctx.size() || b->gen[d] != c->flags_gen) b->ftable_ok = false;
+        // a lattice whose kernels stopped (or started) reading its wave summary: the table
+        // must (or may) hold its XMIX and WRAP face cells
+        if (b->wc.size() != b->ctx.size() || b->wc[d] != (uint8_t)uses_wave_classes(c)) b->ftable_ok = false;
     }
     if (!b->ftable_ok) {
         if (int e = batch_face_table(b)) return e;

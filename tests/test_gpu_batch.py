@@ -44,6 +44,29 @@ def test_batch_equals_single_lattices(q, storage):
                 assert np.array_equal(o[k], r[k]), (grid, chunk, d, k)
 
 
+@pytest.mark.parametrize("storage", [lb.STORE_F32, lb.STORE_F16DEV])
+def test_batch_face_table_follows_the_lattices_wave_summary_use(storage):
+    # the face table leaves out the face cells that XMIX and WRAP waves run themselves; a
+    # lattice whose launch option turns its wave summary off (16 = every wave reads its
+    # flags first) needs them back in the table, and one that turns it on again needs them out
+    cs = cases.ensemble(4, edges=(24, 40))
+    ref = [run_hip(c, 20, storage=storage) for c in cs]
+    with lb.Batch([(c.nx, c.ny, c.nz, c.omega0) for c in cs], q=19, storage=storage, sgs=cs[0].sgs) as b:
+        for c, lat in zip(cs, b.lattice):
+            c.upload_to(lat)
+        params = [c.params for c in cs]
+        b.lattice[1].set_launch(256, 16)
+        b.step(7, params)
+        b.lattice[1].set_launch(256, 0)
+        b.lattice[2].set_launch(256, 16)
+        b.step(6, params)
+        b.lattice[2].set_launch(256, 0)
+        b.step(7, params)
+        for d, (lat, r) in enumerate(zip(b.lattice, ref)):
+            for k in KEYS:
+                assert np.array_equal(lat.download(k), r[k]), (d, k)
+
+
 def test_batch_refuses_a_lattice_stepped_on_its_own():
     cs = cases.ensemble(2, edges=(8, 10))
     with lb.Batch([(c.nx, c.ny, c.nz, c.omega0) for c in cs], q=19, sgs=cs[0].sgs) as b:
