@@ -363,13 +363,32 @@ template <bool Interior, typename Real, int Q>
 __device__ inline Real wale_tau(const StepArgs<Real, Q> &a, long ic, int i, int j, int k, Real u, Real v, Real w)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    if (Interior && i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && k > 0 && k < nz - 1) {
+    if (Interior && Q == 27 && i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && k > 0 && k < nz - 1) {
         // all six neighbours exist: central differences only, as grad_comp takes them there
         const uint32_t o = (uint32_t)ic * (uint32_t)sizeof(Real);
         const uint32_t dx = (uint32_t)sizeof(Real), dy = (uint32_t)nx * dx, dz = (uint32_t)ny * dy;
         const Real *U = a.u_o, *V = a.v_o, *W = a.w_o;
         return wale_from_grad<Real>(cdiff(U, o, dx), cdiff(U, o, dy), cdiff(U, o, dz), cdiff(V, o, dx), cdiff(V, o, dy),
                                     cdiff(V, o, dz), cdiff(W, o, dx), cdiff(W, o, dy), cdiff(W, o, dz));
+    }
+    if (Interior && Q == 19 && nx >= 3 && j > 0 && j < ny - 1 && k > 0 && k < nz - 1) {
+        // (D3Q19, whose waves of x-face and interior cells run together) the y and z
+        // neighbours exist: central differences there, as grad_comp takes them; along x
+        // central differences inside and, on an x face, grad_comp's one-sided difference with
+        // the cell's own new velocity -- as a per-lane choice, so that such a wave runs one
+        // path.  The x neighbours read on a face (the previous row's last or the next row's
+        // first cell) exist, j being interior, and are not used there.
+        const uint32_t o = (uint32_t)ic * (uint32_t)sizeof(Real);
+        const uint32_t dx = (uint32_t)sizeof(Real), dy = (uint32_t)nx * dx, dz = (uint32_t)ny * dy;
+        const Real *U = a.u_o, *V = a.v_o, *W = a.w_o;
+        const bool xlo = (i == 0), xhi = (i == nx - 1);
+        auto ddx = [&](const Real *P, Real self) {
+            const char *b = (const char *)P;
+            const Real p = *(const Real *)(b + (size_t)(o + dx)), m = *(const Real *)(b + (size_t)(o - dx));
+            return xlo ? p - self : (xhi ? self - m : Real(0.5) * (p - m));
+        };
+        return wale_from_grad<Real>(ddx(U, u), cdiff(U, o, dy), cdiff(U, o, dz), ddx(V, v), cdiff(V, o, dy),
+                                    cdiff(V, o, dz), ddx(W, w), cdiff(W, o, dy), cdiff(W, o, dz));
     }
     Real dudx = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 0);
     Real dudy = grad_comp<Real>(a.u_o, u, ic, i, j, k, nx, ny, nz, 1);
