@@ -150,22 +150,22 @@ __device__ inline uint32_t ordered_block(uint32_t g, uint32_t G, uint32_t tune)
 
 // pull source for a face cell (kernel.cl:328-575 upwindID_B): wrap across non-wall faces,
 // -1 for a link through a wall face.
-__device__ inline long face_upwind(int q, int i, int j, int k, int nx, int ny, int nz, int b1, int b2, int b3)
+// Formed from the interior source ic - c_q in 32-bit arithmetic: a crossed periodic face moves
+// the source by a whole row (x), plane (y) or lattice (z), c_x nx, c_y nx ny or c_z nx ny nz
+// cells; unsigned wrap-around is exact because the result lies in the lattice.
+__device__ inline long face_upwind(int q, long ic, int i, int j, int k, int nx, int ny, int nz, int b1, int b2, int b3)
 {
-    int si = i - L19::cx[q], sj = j - L19::cy[q], sk = k - L19::cz[q];
-    if (si < 0 || si >= nx) {
-        if (is_wall_code(b1)) return -1;
-        si = (si < 0) ? nx - 1 : 0;
-    }
-    if (sj < 0 || sj >= ny) {
-        if (is_wall_code(b2)) return -1;
-        sj = (sj < 0) ? ny - 1 : 0;
-    }
-    if (sk < 0 || sk >= nz) {
-        if (is_wall_code(b3)) return -1;
-        sk = (sk < 0) ? nz - 1 : 0;
-    }
-    return (long)si + (long)nx * ((long)sj + (long)ny * sk);
+    const int cx = L19::cx[q], cy = L19::cy[q], cz = L19::cz[q];
+    const bool crx = cx == 1 ? i == 0 : (cx == -1 ? i == nx - 1 : false);
+    const bool cry = cy == 1 ? j == 0 : (cy == -1 ? j == ny - 1 : false);
+    const bool crz = cz == 1 ? k == 0 : (cz == -1 ? k == nz - 1 : false);
+    if ((crx && is_wall_code(b1)) || (cry && is_wall_code(b2)) || (crz && is_wall_code(b3))) return -1;
+    const uint32_t nxny = (uint32_t)nx * (uint32_t)ny;
+    uint32_t s = (uint32_t)ic - (uint32_t)cx - (uint32_t)cy * (uint32_t)nx - (uint32_t)cz * nxny;
+    if (crx) s += (uint32_t)cx * (uint32_t)nx;
+    if (cry) s += (uint32_t)cy * nxny;
+    if (crz) s += (uint32_t)cz * nxny * (uint32_t)nz;
+    return (long)s;
 }
 
 // kernel.cl:105-326 reflectOrMirrorQ as a table lookup: bounce-back partner, with
@@ -665,7 +665,7 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
 #pragma unroll
     for (int q = 1; q < 19; q++) {
         if (xonly && L19::cx[q] != xs) continue;
-        long src = face_upwind(q, i, j, k, nx, ny, nz, b1, b2, b3);
+        long src = face_upwind(q, ic, i, j, k, nx, ny, nz, b1, b2, b3);
         if (src >= 0) {
             ft[q] = Codec<S, Real, 19>::load(fs + (long)q * FS + src, q, w19<Real>(q));
         } else {
@@ -876,7 +876,7 @@ __global__ void aa_swap19_kernel(uint32_t N, int nx, int ny, int nz, FastDiv dnx
     for (int p = 1; p < 19; p += 2) {
         const int po = p + 1;
         // the cell x + c_p: the pull source of direction opp(p)
-        const long nb = (fl & F_FACE) ? face_upwind(po, i, j, k, nx, ny, nz, b1, b2, b3)
+        const long nb = (fl & F_FACE) ? face_upwind(po, (long)icu, i, j, k, nx, ny, nz, b1, b2, b3)
                                       : (long)icu + L19::cx[p] + L19::cy[p] * sy + L19::cz[p] * sz;
         if (nb < 0 || (flag[nb] & F_SOLID)) continue;
         S *x = f + (size_t)p * FS + icu;
