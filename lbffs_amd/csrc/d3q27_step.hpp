@@ -79,23 +79,23 @@ __host__ __device__ constexpr int c27(int q)
     return DIR == 0 ? L27::cx[q] : (DIR == 1 ? L27::cy[q] : L27::cz[q]);
 }
 
-// K27:530-549 upwindID_B: pull source of a face cell, -1 through a wall face
-__device__ inline long face_upwind27(int q, int i, int j, int k, int nx, int ny, int nz, int b1, int b2, int b3)
+// K27:530-549 upwindID_B: pull source of a face cell, -1 through a wall face.  Formed from
+// the interior source ic - c_q in 32-bit arithmetic as the D3Q19 face_upwind: a crossed
+// periodic face moves it by a whole row, plane or lattice (unsigned wrap-around is exact,
+// the result lying in the lattice).
+__device__ inline long face_upwind27(int q, long ic, int i, int j, int k, int nx, int ny, int nz, int b1, int b2, int b3)
 {
-    int si = i - L27::cx[q], sj = j - L27::cy[q], sk = k - L27::cz[q];
-    if (si < 0 || si >= nx) {
-        if (is_wall_code(b1)) return -1;
-        si = (si < 0) ? nx - 1 : 0;
-    }
-    if (sj < 0 || sj >= ny) {
-        if (is_wall_code(b2)) return -1;
-        sj = (sj < 0) ? ny - 1 : 0;
-    }
-    if (sk < 0 || sk >= nz) {
-        if (is_wall_code(b3)) return -1;
-        sk = (sk < 0) ? nz - 1 : 0;
-    }
-    return (long)si + (long)nx * ((long)sj + (long)ny * sk);
+    const int cx = L27::cx[q], cy = L27::cy[q], cz = L27::cz[q];
+    const bool crx = cx == 1 ? i == 0 : (cx == -1 ? i == nx - 1 : false);
+    const bool cry = cy == 1 ? j == 0 : (cy == -1 ? j == ny - 1 : false);
+    const bool crz = cz == 1 ? k == 0 : (cz == -1 ? k == nz - 1 : false);
+    if ((crx && is_wall_code(b1)) || (cry && is_wall_code(b2)) || (crz && is_wall_code(b3))) return -1;
+    const uint32_t nxny = (uint32_t)nx * (uint32_t)ny;
+    uint32_t s = (uint32_t)ic - (uint32_t)cx - (uint32_t)cy * (uint32_t)nx - (uint32_t)cz * nxny;
+    if (crx) s += (uint32_t)cx * (uint32_t)nx;
+    if (cry) s += (uint32_t)cy * nxny;
+    if (crz) s += (uint32_t)cz * nxny * (uint32_t)nz;
+    return (long)s;
 }
 
 // Zou-He wall density numerator of a face: tangential populations plus twice those
@@ -215,7 +215,7 @@ __device__ __forceinline__ void face_cell27(const StepArgs<Real, 27> &a, Real *f
     ft[0] = Codec<S, Real, 27>::load(fs + ic, 0, w27<Real>(0));
 #pragma unroll
     for (int q = 1; q < 27; q++) {
-        const long src = face_upwind27(q, i, j, k, nx, ny, nz, b1, b2, b3);
+        const long src = face_upwind27(q, ic, i, j, k, nx, ny, nz, b1, b2, b3);
         if (src >= 0) {
             ft[q] = Codec<S, Real, 27>::load(fs + (long)q * FS + src, q, w27<Real>(q));
         } else {
