@@ -275,6 +275,13 @@ int lbffs_batch_set_launch(lbffs_batch *batch, int grid, int graph_chunk);
 /* nsteps steps of every lattice; params holds one entry per lattice */
 int lbffs_batch_step(lbffs_batch *batch, int nsteps, const lbffs_step_params *params);
 int lbffs_batch_last_step_ms(lbffs_batch *batch, float *ms);
+/* Time the launch shapes (grid 0 and persistent grids of 2048, 4096 and 8192 work-groups
+ * where the table has more blocks; graph_chunk 0 and 16) over trial_steps steps each, keep
+ * the fastest as lbffs_batch_set_launch would, and put every lattice's populations, moments
+ * and step counters back as they were.  Output pointers may be null.  Like the batch itself
+ * it has no counterpart in the reference. */
+int lbffs_batch_autotune(lbffs_batch *batch, const lbffs_step_params *params, int trial_steps, int *best_grid,
+                         int *best_chunk, float *best_ms);
 int lbffs_batch_destroy(lbffs_batch *batch);
 
 /* Wait for queued device work. */

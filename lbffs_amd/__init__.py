@@ -119,6 +119,8 @@ def lib():
         L.lbffs_batch_set_launch.argtypes = [vp, ip, ip]
         L.lbffs_batch_step.argtypes = [vp, ip, C.POINTER(StepParams)]
         L.lbffs_batch_last_step_ms.argtypes = [vp, C.POINTER(C.c_float)]
+        L.lbffs_batch_autotune.argtypes = [vp, C.POINTER(StepParams), ip, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_float)]
         L.lbffs_batch_destroy.argtypes = [vp]
         fl = C.c_float
         L.lbffs_geom_solid.argtypes = [ip, vp, vp, vp, ip, ip, ip, ip, vp]
@@ -141,7 +143,7 @@ def lib():
                   "lbffs_autotune", "lbffs_set_launch",
                   "lbffs_ibm_set_markers", "lbffs_ibm_set_motion", "lbffs_ibm_download", "lbffs_batch_create",
                   "lbffs_batch_context", "lbffs_batch_set_launch", "lbffs_batch_step", "lbffs_batch_last_step_ms",
-                  "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf", "lbffs_geom_qlinks", "lbffs_geom_qlinks_ctx",
+                  "lbffs_batch_autotune", "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf", "lbffs_geom_qlinks", "lbffs_geom_qlinks_ctx",
                   "lbffs_case_sdf_near_bwalls",
                   "lbffs_output_macros", "lbffs_out_write_vtk", "lbffs_out_write_restart", "lbffs_out_read_restart",
                   "lbffs_out_stats_compute", "lbffs_out_coeffs_fp64", "lbffs_out_accumulate_mean"):
@@ -351,6 +353,18 @@ class Batch:
         ms = C.c_float()
         _check(lib().lbffs_batch_last_step_ms(self._h, C.byref(ms)), "lbffs_batch_last_step_ms")
         return ms.value
+
+    def autotune(self, params, trial_steps=10):
+        """Time the batch's launch shapes, keep the fastest, leave every lattice unchanged.
+
+        Returns (grid, graph_chunk, ms per step) of the kept shape."""
+        n = len(self.lattice)
+        plist = params if isinstance(params, (list, tuple)) else [params] * n
+        arr = (StepParams * n)(*[_params(**p) for p in plist])
+        g, ch, ms = C.c_int(), C.c_int(), C.c_float()
+        _check(lib().lbffs_batch_autotune(self._h, arr, int(trial_steps), C.byref(g), C.byref(ch), C.byref(ms)),
+               "lbffs_batch_autotune")
+        return g.value, ch.value, ms.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <memory>
 #include <mutex>
 
 #include "../../include/lbffs.h"
@@ -1167,6 +1168,71 @@ void batch_free(lbffs_batch *b)
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
+// a device copy of everything a step writes (the current populations, moments, relaxation
+// times, wall forces, marker forcing and markers) and the step counters, so that a tuner can
+// run trial steps and leave the lattice as it found it
+struct StateCopy {
+    lbffs_ctx *c = nullptr;
+    void *f = nullptr, *r = nullptr, *m = nullptr;
+    int cur = 0, mcur = 0, aa_odd = 0;
+    uint32_t nstep = 0;
+
+    StateCopy() = default;
+    StateCopy(const StateCopy &) = delete;
+    StateCopy &operator=(const StateCopy &) = delete;
+    ~StateCopy()
+    {
+        for (void *p : {f, r, m})
+            if (p) (void)hipFree(p);
+    }
+    size_t fbytes() const { return (size_t)c->d.q * c->fstride * c->store_size; }
+    size_t rbytes() const { return (size_t)c->N * c->real_size; }
+    size_t mbytes() const { return (size_t)3 * (size_t)c->n_markers * c->real_size; }
+    void fields(void *k[11], int mc) const
+    {
+        void *a[11] = {c->rho[mc], c->u[mc], c->v[mc], c->w[mc], c->tau, c->Fwx, c->Fwy, c->Fwz, c->Gx, c->Gy, c->Gz};
+        for (int i = 0; i < 11; i++) k[i] = a[i];
+    }
+    // allocate and enqueue the copies on the lattice's stream
+    int save(lbffs_ctx *ctx)
+    {
+        c = ctx;
+        cur = c->cur;
+        mcur = c->mcur;
+        aa_odd = c->aa_odd;
+        nstep = c->nstep;
+        if (int e = dalloc(&f, fbytes())) return e;
+        if (int e = dalloc(&r, 11 * rbytes())) return e;
+        if (c->n_markers > 0)
+            if (int e = dalloc(&m, mbytes())) return e;
+        void *k[11];
+        fields(k, mcur);
+        hipError_t he = hipMemcpyAsync(f, c->f[cur], fbytes(), hipMemcpyDeviceToDevice, c->stream);
+        for (int i = 0; i < 11 && he == hipSuccess; i++)
+            he = hipMemcpyAsync((char *)r + i * rbytes(), k[i], rbytes(), hipMemcpyDeviceToDevice, c->stream);
+        if (m && he == hipSuccess) he = hipMemcpyAsync(m, c->markers, mbytes(), hipMemcpyDeviceToDevice, c->stream);
+        if (he != hipSuccess) return fail(LBFFS_E_DEVICE, std::string("state copy: ") + hipGetErrorString(he));
+        return 0;
+    }
+    // put the step counters back and enqueue the copies back
+    hipError_t restore()
+    {
+        c->cur = cur;
+        c->mcur = mcur;
+        c->aa_odd = aa_odd;
+        c->nstep = nstep;
+        c->g_dirty = true;  // the restored G are the saved ones, not those of the last list
+        c->timed = false;
+        void *k[11];
+        fields(k, mcur);
+        hipError_t he = hipMemcpyAsync(c->f[cur], f, fbytes(), hipMemcpyDeviceToDevice, c->stream);
+        for (int i = 0; i < 11 && he == hipSuccess; i++)
+            he = hipMemcpyAsync(k[i], (char *)r + i * rbytes(), rbytes(), hipMemcpyDeviceToDevice, c->stream);
+        if (m && he == hipSuccess) he = hipMemcpyAsync(c->markers, m, mbytes(), hipMemcpyDeviceToDevice, c->stream);
+        return he;
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -1338,32 +1404,11 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
     if (int e = aa_restore(c)) return e;  // the saved array then restarts at a first half-step
     HIP_TRY(hipStreamSynchronize(c->stream));
     // save everything a step writes, so that tuning leaves the lattice state unchanged
-    const size_t n = (size_t)c->N, rs = c->real_size;
-    const size_t fbytes = (size_t)c->d.q * c->fstride * c->store_size;
-    void *saved_f = nullptr, *saved_r = nullptr;
-    if (int e = dalloc(&saved_f, fbytes)) return e;
-    if (int e = dalloc(&saved_r, 11 * n * rs)) {
-        (void)hipFree(saved_f);
-        return e;
-    }
-    void *keep[11] = {c->rho[c->mcur], c->u[c->mcur], c->v[c->mcur], c->w[c->mcur], c->tau,
-                      c->Fwx, c->Fwy, c->Fwz, c->Gx, c->Gy, c->Gz};
-    const size_t mbytes = (size_t)3 * (size_t)c->n_markers * rs;
-    void *saved_m = nullptr;
-    if (c->n_markers > 0) {
-        if (int e = dalloc(&saved_m, mbytes)) {
-            (void)hipFree(saved_f);
-            (void)hipFree(saved_r);
-            return e;
-        }
-    }
-    const int cur0 = c->cur, mcur0 = c->mcur, block0 = c->block, tune0 = c->tune;
-    const uint32_t nstep0 = c->nstep;
+    StateCopy saved;
+    if (int e = saved.save(c)) return e;
+    const int block0 = c->block, tune0 = c->tune;
     int rc = 0;
-    hipError_t he = hipMemcpyAsync(saved_f, c->f[cur0], fbytes, hipMemcpyDeviceToDevice, c->stream);
-    for (int k = 0; k < 11 && he == hipSuccess; k++)
-        he = hipMemcpyAsync((char *)saved_r + k * n * rs, keep[k], n * rs, hipMemcpyDeviceToDevice, c->stream);
-    if (saved_m && he == hipSuccess) he = hipMemcpyAsync(saved_m, c->markers, mbytes, hipMemcpyDeviceToDevice, c->stream);
+    hipError_t he = hipSuccess;
     const int blocks[3] = {64, 128, 256};
     // launch options: the work-group order (XCD slabs, XCD chunks or the hardware's), the
     // face cells in the same waves as their neighbours, in a second launch or in the first
@@ -1410,19 +1455,9 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
         }
     }
     // restore the state
-    c->cur = cur0;
-    c->mcur = mcur0;
-    c->nstep = nstep0;
-    c->aa_odd = 0;
-    if (he == hipSuccess) he = hipMemcpyAsync(c->f[cur0], saved_f, fbytes, hipMemcpyDeviceToDevice, c->stream);
-    for (int k = 0; k < 11 && he == hipSuccess; k++)
-        he = hipMemcpyAsync(keep[k], (char *)saved_r + k * n * rs, n * rs, hipMemcpyDeviceToDevice, c->stream);
-    if (saved_m && he == hipSuccess) he = hipMemcpyAsync(c->markers, saved_m, mbytes, hipMemcpyDeviceToDevice, c->stream);
+    const hipError_t hr = saved.restore();
+    if (he == hipSuccess) he = hr;
     if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
-    (void)hipFree(saved_f);
-    (void)hipFree(saved_r);
-    if (saved_m) (void)hipFree(saved_m);
-    c->g_dirty = true;  // the restored G are the pre-tuning ones, not those of the last list
     if (rc) {
         c->block = block0;
         c->tune = tune0;
@@ -1730,6 +1765,65 @@ int lbffs_batch_last_step_ms(lbffs_batch *b, float *ms)
     if (!b->timed) return fail(LBFFS_E_STATE, "no step has run");
     HIP_TRY(hipEventSynchronize(b->ev1));
     HIP_TRY(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return 0;
+}
+
+int lbffs_batch_autotune(lbffs_batch *b, const lbffs_step_params *params, int trial_steps, int *best_grid,
+                         int *best_chunk, float *best_ms)
+{
+    if (!b || !params) return fail(LBFFS_E_INVALID, "null argument");
+    if (trial_steps < 1) return fail(LBFFS_E_INVALID, "trial_steps < 1");
+    // checks every lattice, packs changed flags and rebuilds the face table; steps nothing
+    if (int e = lbffs_batch_step(b, 0, params)) return e;
+    const size_t n = b->ctx.size();
+    std::unique_ptr<StateCopy[]> saved(new StateCopy[n]);
+    for (size_t d = 0; d < n; d++)
+        if (int e = saved[d].save(b->ctx[d])) return e;
+    const int grid0 = b->grid, chunk0 = b->graph_chunk;
+    // one work-group per 256-cell block, or persistent grids of 8, 16 and 32 work-groups per
+    // CU where the table has more blocks than that; plain launches or captured 16-step chunks
+    const int grids[4] = {0, 2048, 4096, 8192};
+    const int chunks[2] = {0, 16};
+    float bms = 1e30f;
+    int bg = grid0, bc = chunk0, rc = 0;
+    for (int gi = 0; gi < 4 && !rc; gi++) {
+        if (grids[gi] > 0 && (uint32_t)grids[gi] >= b->nblocks) continue;
+        for (int ci = 0; ci < 2 && !rc; ci++) {
+            b->grid = grids[gi];
+            b->graph_chunk = chunks[ci];
+            batch_drop_graphs(b);  // a captured graph keeps the grid it was captured with
+            // two warm calls of odd length capture the graphs of both step parities
+            const int warm = chunks[ci] + 1;
+            rc = lbffs_batch_step(b, warm, params);
+            if (!rc) rc = lbffs_batch_step(b, warm, params);
+            if (!rc) rc = lbffs_batch_step(b, trial_steps, params);
+            float ms = 0.f;
+            if (!rc) rc = lbffs_batch_last_step_ms(b, &ms);
+            if (!rc && ms / trial_steps < bms) {
+                bms = ms / trial_steps;
+                bg = grids[gi];
+                bc = chunks[ci];
+            }
+        }
+    }
+    hipError_t he = hipStreamSynchronize(b->stream);
+    for (size_t d = 0; d < n; d++) {
+        const hipError_t hr = saved[d].restore();
+        if (he == hipSuccess) he = hr;
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(b->stream);
+    b->timed = false;
+    batch_drop_graphs(b);
+    if (rc || he != hipSuccess) {
+        b->grid = grid0;
+        b->graph_chunk = chunk0;
+        return rc ? rc : fail(LBFFS_E_DEVICE, std::string("batch autotune: ") + hipGetErrorString(he));
+    }
+    b->grid = bg;
+    b->graph_chunk = bc;
+    if (best_grid) *best_grid = bg;
+    if (best_chunk) *best_chunk = bc;
+    if (best_ms) *best_ms = bms;
     return 0;
 }
 

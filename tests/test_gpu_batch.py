@@ -75,3 +75,24 @@ def test_batch_refuses_a_lattice_stepped_on_its_own():
         b.lattice[0].step(1, **cs[0].params)
         with pytest.raises(lb.LbffsError, match="parity"):
             b.step(1, [c.params for c in cs])
+
+
+@pytest.mark.parametrize("storage", [lb.STORE_F32, lb.STORE_F16DEV])
+def test_batch_autotune_leaves_every_lattice_unchanged(storage):
+    # 12857 blocks of 256 cells: every persistent grid (2048, 4096, 8192) is tried; the
+    # 16-bit form also checks that the step index feeding the stochastic rounding is put back
+    cs = cases.ensemble(6, edges=(48, 96))
+    ref = [run_hip(c, 20, storage=storage) for c in cs]
+    with lb.Batch([(c.nx, c.ny, c.nz, c.omega0) for c in cs], q=19, storage=storage, sgs=cs[0].sgs) as b:
+        for c, lat in zip(cs, b.lattice):
+            c.upload_to(lat)
+        params = [c.params for c in cs]
+        b.step(7, params)
+        grid, chunk, ms = b.autotune(params, 3)
+        assert grid in (0, 2048, 4096, 8192) and chunk in (0, 16) and ms > 0
+        b.step(13, params)
+        for d, (lat, r) in enumerate(zip(b.lattice, ref)):
+            for k in KEYS:
+                assert np.array_equal(lat.download(k), r[k]), (grid, chunk, d, k)
+        with pytest.raises(lb.LbffsError, match="trial_steps"):
+            b.autotune(params, 0)
