@@ -313,11 +313,15 @@ int lbffs_probe_stream(int kind, int unroll, int nt, int block, int grid, size_t
 /* Diagnostics, no counterpart in the reference: the memory skeleton of one lattice step --
  * every cell of an nx*ny*nz lattice (a multiple of 256 cells) pulls its q (19 or 27)
  * populations of store_bytes (4 or 2) with the step's offsets and stores them to its own
- * slots of a second array, with no collision.  variant 0: the lattice offsets (x-shifted
- * runs straddle cache lines); 1: the x component of every offset dropped (aligned runs:
- * the ideal pull); 2: x-shifted directions loaded aligned and moved one lane by a DPP wave
- * shift, the lane leaving the wave loading its value alone; 3: as 2, with that lane's value
- * passed between the four waves of a work-group through LDS. */
+ * slots of a second array, with no collision.  variant 0: the engine's layout and the
+ * lattice offsets (4-byte values in population blocks of 8192 cells, each holding its q
+ * directions one after another; 2-byte values as variant 4).  The others lay the
+ * populations out as q arrays of the whole lattice (direction-major): 4 the lattice
+ * offsets (x-shifted runs
+ * straddle cache lines); 1 the x component of every offset dropped (aligned runs: the ideal
+ * pull); 2 x-shifted directions loaded aligned and moved one lane by a DPP wave shift, the
+ * lane leaving the wave loading its value alone; 3 as 2, with that lane's value passed
+ * between the four waves of a work-group through LDS. */
 int lbffs_probe_lattice(int q, int store_bytes, int variant, int nx, int ny, int nz, int reps, float *ms_per_pass);
 
 #ifdef __cplusplus

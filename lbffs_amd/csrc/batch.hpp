@@ -63,7 +63,7 @@ __host__ inline unsigned batch_grid(unsigned grid, uint32_t n)
 }
 
 template <typename S, typename Real, int SGS, int Q, bool IBB, int PASS, int COLL = COLL_CUMULANT>
-__global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Q == 27 ? kBatch27MinWaves<Real> : 1, 8)))
+__global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Q == 27 ? kBatch27MinWaves<Real> : (IBB || PASS != 1 || sizeof(Real) == 8 ? 1 : 7), 8)))
     batch_step_kernel(const StepArgs<Real, Q> *__restrict__ args, const BatchBlock *__restrict__ table,
                       uint32_t nblocks, const uint32_t *__restrict__ cells)
 {
@@ -87,6 +87,7 @@ __global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_e
 #define CELL_AA 0
 #define CELL_SKIP_FACE (PASS == 1)
 #define CELL_PURE_OK (PASS == 1)
+#define CELL_ALIGNED (PASS == 1)
 #define CELL_COLL COLL
             if constexpr (Q == 27) {
 #include "d3q27_cell.inc"
@@ -94,6 +95,7 @@ __global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_e
 #include "d3q19_cell.inc"
             }
 #undef CELL_COLL
+#undef CELL_ALIGNED
 #undef CELL_PURE_OK
 #undef CELL_SKIP_FACE
 #undef CELL_AA

@@ -42,7 +42,8 @@ __host__ __device__ constexpr Real w19(int q)
 template <typename Real, int Q>
 struct StepArgs {
     uint32_t N;
-    uint32_t fstride;  // elements between two population arrays (>= N, padded: see the API)
+    uint32_t fstride;  // 16-bit values: elements between two direction arrays (pop_at)
+    int wave_runs;     // 1: a wave of the cell order holds one aligned run of 64 cells
     int nx, ny, nz;
     FastDiv dnx, dny;
     int off[Q];  // interior pull offset: source = ic - off[q]
@@ -655,22 +656,21 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
                                        Real omega, bool xonly)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const long FS = a.fstride;
     const S *fs = (const S *)a.fsrc;
     const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
     const bool sym = (b1 == BC_SYMMETRY || b2 == BC_SYMMETRY || b3 == BC_SYMMETRY);
     Real Fx = 0, Fy = 0, Fz = 0;  // wall momentum of this step's domain-face links
     const int xs = (i == 0) ? 1 : -1;
-    if (!xonly) ft[0] = Codec<S, Real, 19>::load(fs + ic, 0, w19<Real>(0));
+    if (!xonly) ft[0] = Codec<S, Real, 19>::load(fs + pop_at<19, S>(0, (uint32_t)ic, a.fstride), 0, w19<Real>(0));
 #pragma unroll
     for (int q = 1; q < 19; q++) {
         if (xonly && L19::cx[q] != xs) continue;
         long src = face_upwind(q, ic, i, j, k, nx, ny, nz, b1, b2, b3);
         if (src >= 0) {
-            ft[q] = Codec<S, Real, 19>::load(fs + (long)q * FS + src, q, w19<Real>(q));
+            ft[q] = Codec<S, Real, 19>::load(fs + pop_at<19, S>(q, (uint32_t)src, a.fstride), q, w19<Real>(q));
         } else {
             int qb = sym ? mirror_q(q, b1, b2, b3) : L19::opp(q);
-            ft[q] = Codec<S, Real, 19>::load(fs + (long)qb * FS + ic, qb, w19<Real>(qb));
+            ft[q] = Codec<S, Real, 19>::load(fs + pop_at<19, S>(qb, (uint32_t)ic, a.fstride), qb, w19<Real>(qb));
             Fx += Real(-2) * ft[q] * Real(L19::cx[q]);
             Fy += Real(-2) * ft[q] * Real(L19::cy[q]);
             Fz += Real(-2) * ft[q] * Real(L19::cz[q]);
@@ -779,11 +779,10 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
 #pragma unroll
         for (int q = 1; q < 19; q++) {
             const int cx = L19::cx[q], cy = L19::cy[q], cz = L19::cz[q];
-            const S *fq = fs + (long)q * FS;
 #define OUTF(cond, inner, un)                                                                          \
     if (cond) {                                                                                        \
-        const Real fi = Codec<S, Real, 19>::load(fq + (inner), q, w19<Real>(q));                      \
-        const Real fc = Codec<S, Real, 19>::load(fq + ic, q, w19<Real>(q));                           \
+        const Real fi = Codec<S, Real, 19>::load(fs + pop_at<19, S>(q, (uint32_t)(inner), a.fstride), q, w19<Real>(q)); \
+        const Real fc = Codec<S, Real, 19>::load(fs + pop_at<19, S>(q, (uint32_t)ic, a.fstride), q, w19<Real>(q)); \
         ft[q] = (is3 - (un)) * fi + (Real(1) - (is3 - (un))) * fc;                                      \
     }
             OUTF(i == 0 && b1 == BC_OUTLET && cx == 1, ic + 1, uo)
@@ -797,9 +796,8 @@ __device__ __forceinline__ void face_cell(const Step19Args<Real> &a, Real *ft, u
     }
 }
 
-// IBB selects the interpolated bounce-back at compile time: its loop raises the register
-// count of the whole kernel from 63 to 72 VGPRs (7 instead of 8 waves per SIMD), which the
-// 16-bit forms, bound by instruction issue, feel.  AA: 0 two-lattice streaming, 1 and 2
+// IBB selects the interpolated bounce-back at compile time: its loop adds registers to the
+// whole kernel, which the kernels without it should not carry.  AA: 0 two-lattice streaming, 1 and 2
 // the two half-steps of in-place streaming (the cell body describes them).  PASS: 0 every
 // cell; 1 every cell but the lattice-face cells; 2 the face cells of the list a.fcells.  A
 // wave that holds face and interior cells runs both the face path and the interior pull;
@@ -818,8 +816,15 @@ __device__ inline uint32_t face_blocks(const A &a)
     return merged_face_blocks(a.nface, blockDim.x);
 }
 
+// Waves per SIMD the step kernel is held to: eight (64 VGPRs), which the compiler reaches
+// without spilling by interleaving each direction's pull address with its load; seven for
+// the FP32 WALE kernel with interpolated bounce-back, which spills at eight; FP64 as the
+// compiler chooses.
+template <typename S, typename Real, int SGS, bool IBB>
+constexpr int kStep19MinWaves = sizeof(Real) == 8 ? 1 : ((IBB && SGS == 0 && sizeof(S) == 4) ? 7 : 8);
+
 template <typename S, typename Real, int SGS, bool IBB = false, int AA = 0, int PASS = 0>
-__global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStep19MinWaves<S, Real, SGS, IBB>, 8))) step19_kernel(const Step19Args<Real> a)
 {
     const uint32_t nfb = (PASS == 3) ? face_blocks(a) : 0u;
     const bool face_wg = (PASS == 2) || (PASS == 3 && blockIdx.x < nfb);
@@ -842,7 +847,9 @@ __global__ void __launch_bounds__(256) step19_kernel(const Step19Args<Real> a)
 #define CELL_AA AA
 #define CELL_SKIP_FACE (PASS == 1 || (PASS == 3 && !face_wg))
 #define CELL_PURE_OK (AA == 0 && !face_wg)
+#define CELL_ALIGNED (!face_wg && a.wave_runs)
 #include "d3q19_cell.inc"
+#undef CELL_ALIGNED
 #undef CELL_PURE_OK
 #undef CELL_SKIP_FACE
 #undef CELL_AA
@@ -879,8 +886,8 @@ __global__ void aa_swap19_kernel(uint32_t N, int nx, int ny, int nz, FastDiv dnx
         const long nb = (fl & F_FACE) ? face_upwind(po, (long)icu, i, j, k, nx, ny, nz, b1, b2, b3)
                                       : (long)icu + L19::cx[p] + L19::cy[p] * sy + L19::cz[p] * sz;
         if (nb < 0 || (flag[nb] & F_SOLID)) continue;
-        S *x = f + (size_t)p * FS + icu;
-        S *y = f + (size_t)po * FS + nb;
+        S *x = f + pop_at<19, S>(p, icu, FS);
+        S *y = f + pop_at<19, S>(po, (uint32_t)nb, FS);
         const S tx = *x;
         *x = *y;
         *y = tx;

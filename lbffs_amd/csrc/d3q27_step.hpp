@@ -208,16 +208,15 @@ __device__ __forceinline__ void face_cell27(const StepArgs<Real, 27> &a, Real *f
                                             int k, Real omega)
 {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const long FS = a.fstride;
     const S *fs = (const S *)a.fsrc;
     const int b1 = flag_code(fl, F_B1_SHIFT), b2 = flag_code(fl, F_B2_SHIFT), b3 = flag_code(fl, F_B3_SHIFT);
     Real Fx = 0, Fy = 0, Fz = 0;  // wall momentum of this step's domain-face links
-    ft[0] = Codec<S, Real, 27>::load(fs + ic, 0, w27<Real>(0));
+    ft[0] = Codec<S, Real, 27>::load(fs + pop_at<27, S>(0, (uint32_t)ic, a.fstride), 0, w27<Real>(0));
 #pragma unroll
     for (int q = 1; q < 27; q++) {
         const long src = face_upwind27(q, ic, i, j, k, nx, ny, nz, b1, b2, b3);
         if (src >= 0) {
-            ft[q] = Codec<S, Real, 27>::load(fs + (long)q * FS + src, q, w27<Real>(q));
+            ft[q] = Codec<S, Real, 27>::load(fs + pop_at<27, S>(q, (uint32_t)src, a.fstride), q, w27<Real>(q));
         } else {
             // reflected population; a Symmetry face code mirrors instead, the z test
             // overriding the y test overriding the x test (K27:1133-1145)
@@ -225,7 +224,7 @@ __device__ __forceinline__ void face_cell27(const StepArgs<Real, 27> &a, Real *f
             qb = (b1 == BC_SYMMETRY) ? L27::mirx[q] : qb;
             qb = (b2 == BC_SYMMETRY) ? L27::miry[q] : qb;
             qb = (b3 == BC_SYMMETRY) ? L27::mirz[q] : qb;
-            ft[q] = Codec<S, Real, 27>::load(fs + (long)qb * FS + ic, qb, w27<Real>(qb));
+            ft[q] = Codec<S, Real, 27>::load(fs + pop_at<27, S>(qb, (uint32_t)ic, a.fstride), qb, w27<Real>(qb));
             Fx += Real(-2) * ft[q] * Real(L27::cx[q]);
             Fy += Real(-2) * ft[q] * Real(L27::cy[q]);
             Fz += Real(-2) * ft[q] * Real(L27::cz[q]);
@@ -261,11 +260,10 @@ __device__ __forceinline__ void face_cell27(const StepArgs<Real, 27> &a, Real *f
 #pragma unroll
         for (int q = 1; q < 27; q++) {
             const int cx = L27::cx[q], cy = L27::cy[q], cz = L27::cz[q];
-            const S *fq = fs + (long)q * FS;
 #define OUTF(cond, inner, un)                                                                          \
     if (cond) {                                                                                        \
-        const Real fi = Codec<S, Real, 27>::load(fq + (inner), q, w27<Real>(q));                      \
-        const Real fc = Codec<S, Real, 27>::load(fq + ic, q, w27<Real>(q));                           \
+        const Real fi = Codec<S, Real, 27>::load(fs + pop_at<27, S>(q, (uint32_t)(inner), a.fstride), q, w27<Real>(q)); \
+        const Real fc = Codec<S, Real, 27>::load(fs + pop_at<27, S>(q, (uint32_t)ic, a.fstride), q, w27<Real>(q)); \
         ft[q] = (is3 - (un)) * fi + (Real(1) - (is3 - (un))) * fc;                                      \
     }
             OUTF(i == 0 && b1 == BC_OUTLET && cx == 1, ic + 1, uo)
@@ -659,9 +657,11 @@ step27_kernel(const StepArgs<Real, 27> a)
 #define CELL_SKIP_FACE (PASS == 1 || (PASS == 3 && !face_wg))
 // (the 16-bit split interior pass would spill with the per-wave fast path)
 #define CELL_PURE_OK (!face_wg && !(PASS == 1 && sizeof(S) == 2))
+#define CELL_ALIGNED (!face_wg && a.wave_runs)
 #define CELL_COLL COLL
 #include "d3q27_cell.inc"
 #undef CELL_COLL
+#undef CELL_ALIGNED
 #undef CELL_PURE_OK
 #undef CELL_SKIP_FACE
 #undef CELL_WALE_INTERIOR

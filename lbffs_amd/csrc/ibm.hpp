@@ -142,7 +142,7 @@ __global__ void ibm_spread_move_kernel(int n, Real *xyz, int nx, int ny, int nz,
 // k_Force (K19:3781-3807) on the listed cells; zero_after clears G behind it so that the
 // next step spreads into zeros
 template <typename S, typename Real, int Q>
-__global__ void ibm_force_kernel(const int *list, const int *count, int cap, size_t fstride, S *f, const uint16_t *flag,
+__global__ void ibm_force_kernel(const int *list, const int *count, int cap, size_t fs, S *f, const uint16_t *flag,
                                  const Real *rho, Real *Gx, Real *Gy, Real *Gz, int zero_after)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,7 +163,7 @@ __global__ void ibm_force_kernel(const int *list, const int *count, int cap, siz
         const int cy = Q == 27 ? L27::cy[q] : L19::cy[q];
         const int cz = Q == 27 ? L27::cz[q] : L19::cz[q];
         const Real wqq = wq<Real, Q>(q);
-        S *p = f + (size_t)q * fstride + ic;
+        S *p = f + pop_at<Q, S>(q, ic, fs);
         const Real fq = Codec<S, Real, Q>::load(p, q, wqq);
         Codec<S, Real, Q>::store(p, fq + r3 * wqq * (gx * Real(cx) + gy * Real(cy) + gz * Real(cz)), q, wqq);
     }

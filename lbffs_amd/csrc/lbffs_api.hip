@@ -60,7 +60,8 @@ enum : int {
 struct lbffs_ctx {
     lbffs_desc d;
     long N = 0;
-    size_t fstride = 0;  // elements between two population arrays on the device (>= N)
+    size_t felems = 0;   // elements of one population array on the device (pop_elems)
+    size_t fstride = 0;  // 16-bit values: elements between two direction arrays (pop_at)
     int real_size = 4;   // bytes of the host/compute real type
     int store_size = 4;  // bytes of one stored population
     void *f[2] = {nullptr, nullptr};  // one array (f[0]) with in-place streaming
@@ -207,7 +208,7 @@ __global__ void pack_flags_kernel(int Q, uint32_t N, int nx, int ny, int nz, Fas
     if (e) atomicOr(err, e);
 }
 
-// host-real populations (stride N) -> storage form (stride fs)
+// host-real populations (q-major arrays of N) -> storage form in the device layout (pop_at)
 template <typename S, typename Real, int Q>
 __global__ void encode_kernel(uint32_t N, size_t fs, const Real *src, S *dst)
 {
@@ -215,7 +216,7 @@ __global__ void encode_kernel(uint32_t N, size_t fs, const Real *src, S *dst)
     if (ic >= N) return;
 #pragma unroll
     for (int q = 0; q < Q; q++)
-        Codec<S, Real, Q>::store(dst + (size_t)q * fs + ic, src[(size_t)q * N + ic], q, wq<Real, Q>(q));
+        Codec<S, Real, Q>::store(dst + pop_at<Q, S>(q, ic, fs), src[(size_t)q * N + ic], q, wq<Real, Q>(q));
 }
 
 template <typename S, typename Real, int Q>
@@ -225,7 +226,7 @@ __global__ void decode_kernel(uint32_t N, size_t fs, const S *src, Real *dst)
     if (ic >= N) return;
 #pragma unroll
     for (int q = 0; q < Q; q++)
-        dst[(size_t)q * N + ic] = Codec<S, Real, Q>::load(src + (size_t)q * fs + ic, q, wq<Real, Q>(q));
+        dst[(size_t)q * N + ic] = Codec<S, Real, Q>::load(src + pop_at<Q, S>(q, ic, fs), q, wq<Real, Q>(q));
 }
 
 // density and velocity from the current populations, as the host computes them at an
@@ -242,7 +243,7 @@ __global__ void output_macros_kernel(uint32_t N, size_t fs, const S *f, Real dpd
     Real r = 0, mx = 0, my = 0, mz = 0;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        const Real fq = Codec<S, Real, Q>::load(f + (size_t)q * fs + ic, q, wq<Real, Q>(q));
+        const Real fq = Codec<S, Real, Q>::load(f + pop_at<Q, S>(q, ic, fs), q, wq<Real, Q>(q));
         const Real cx = Real(Q == 27 ? L27::cx[q] : L19::cx[q]);
         const Real cy = Real(Q == 27 ? L27::cy[q] : L19::cy[q]);
         const Real cz = Real(Q == 27 ? L27::cz[q] : L19::cz[q]);
@@ -289,7 +290,7 @@ __global__ void __launch_bounds__(kRedThreads) reduce_kernel(uint32_t N, size_t 
         double rho = 0;
 #pragma unroll
         for (int q = 0; q < Q; q++)
-            rho += (double)Codec<S, Real, Q>::load(f + (size_t)q * fs + ic, q, wq<Real, Q>(q));
+            rho += (double)Codec<S, Real, Q>::load(f + pop_at<Q, S>(q, ic, fs), q, wq<Real, Q>(q));
         acc[6] += rho - 1.0;
         acc[7] += 1.0;
     }
@@ -385,20 +386,6 @@ bool field_affects_flags(int field)
 }
 
 inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
-
-// Elements between two population arrays on the device: the first count >= n whose byte
-// size is 8 KiB past a multiple of 256 KiB.  Arrays whose sizes are large powers of two
-// apart fall into the same cache sets and HBM channels when a cell's 19 or 27 pulls go
-// out together; the 8 KiB skew spreads the directions over 32 slots of a 256 KiB span.
-// (Measured on 256^3 lattices, whose arrays are 2^25 bytes: the 16-bit-storage step
-// ran 27 % slower than on 255 x 256 x 256.)
-size_t population_stride(size_t n, int elem)
-{
-    const size_t span = size_t(1) << 18, skew = 8192;
-    const size_t bytes = n * (size_t)elem;
-    size_t padded = (bytes <= skew) ? skew : ((bytes - skew + span - 1) / span) * span + skew;
-    return padded / (size_t)elem;
-}
 
 // the device address of `scratch` on `device` (the current device), looked up on the first
 // call for that device
@@ -571,7 +558,6 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
 {
     const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
     a.N = (uint32_t)c->N;
-    a.fstride = (uint32_t)c->fstride;
     a.nx = nx;
     a.ny = ny;
     a.nz = nz;
@@ -584,6 +570,7 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
         const int zoff = (nz == 1) ? 0 : cz;  // nz==1: z links stay in the layer (K19:907-989, K27:797-820)
         a.off[q] = cx + nx * cy + nx * ny * zoff;
     }
+    a.fstride = (uint32_t)c->fstride;
     a.flag = c->flag;
     a.u0 = (const Real *)c->u0;
     a.v0 = (const Real *)c->v0;
@@ -629,6 +616,9 @@ void set_args(lbffs_ctx *c, StepArgs<Real, Q> &a, const lbffs_step_params *p)
         a.dplane = make_fastdiv(a.tile_plane);
         if (!ty) a.tune &= ~TUNE_YTILE;
     }
+    // the plain order gives every wave one aligned run of 64 cells; y-tiles do where their
+    // rows are whole waves
+    a.wave_runs = (!(a.tune & TUNE_YTILE) || nx % 64 == 0) ? 1 : 0;
     a.wclass = uses_wave_classes(c) ? c->wclass : nullptr;
 }
 
@@ -784,16 +774,9 @@ int run_steps(lbffs_ctx *c, int nsteps, const lbffs_step_params *p)
 template <typename S, typename Real>
 int upload_f(lbffs_ctx *c, const void *host)
 {
+    // host q-major arrays of N -> a device copy of them -> the storage form in the device
+    // layout (encode_kernel)
     const size_t n = (size_t)c->d.q * c->N;
-    if (sizeof(S) == sizeof(Real)) {
-        // host q-major arrays of N -> device arrays of stride fstride
-        for (int b = 0; b < 2; b++)
-            if (c->f[b])
-                HIP_TRY(hipMemcpy2D(c->f[b], c->fstride * sizeof(Real), host, c->N * sizeof(Real),
-                                    c->N * sizeof(Real), c->d.q, hipMemcpyHostToDevice));
-        c->aa_odd = 0;
-        return 0;
-    }
     void *tmp = nullptr;
     if (int e = dalloc(&tmp, n * sizeof(Real))) return e;
     hipError_t e1 = hipMemcpy(tmp, host, n * sizeof(Real), hipMemcpyHostToDevice);
@@ -821,11 +804,6 @@ int download_f(lbffs_ctx *c, void *host)
 {
     const size_t n = (size_t)c->d.q * c->N;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (sizeof(S) == sizeof(Real)) {
-        HIP_TRY(hipMemcpy2D(host, c->N * sizeof(Real), c->f[c->cur], c->fstride * sizeof(Real), c->N * sizeof(Real),
-                            c->d.q, hipMemcpyDeviceToHost));
-        return 0;
-    }
     void *tmp = nullptr;
     if (int e = dalloc(&tmp, n * sizeof(Real))) return e;
     if (c->d.q == 27)
@@ -1185,7 +1163,7 @@ struct StateCopy {
         for (void *p : {f, r, m})
             if (p) (void)hipFree(p);
     }
-    size_t fbytes() const { return (size_t)c->d.q * c->fstride * c->store_size; }
+    size_t fbytes() const { return c->felems * c->store_size; }
     size_t rbytes() const { return (size_t)c->N * c->real_size; }
     size_t mbytes() const { return (size_t)3 * (size_t)c->n_markers * c->real_size; }
     void fields(void *k[11], int mc) const
@@ -1288,9 +1266,10 @@ int lbffs_create(const lbffs_desc *desc, lbffs_ctx **out)
         delete c;
         return fail(LBFFS_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
-    c->fstride = population_stride(n, c->store_size);
-    if (!e) e = dalloc(&c->f[0], (size_t)desc->q * c->fstride * c->store_size);
-    if (!e && desc->streaming != LBFFS_STREAM_AA) e = dalloc(&c->f[1], (size_t)desc->q * c->fstride * c->store_size);
+    c->felems = pop_elems(desc->q, c->store_size, n);
+    c->fstride = pop_stride(c->store_size, n);
+    if (!e) e = dalloc(&c->f[0], c->felems * c->store_size);
+    if (!e && desc->streaming != LBFFS_STREAM_AA) e = dalloc(&c->f[1], c->felems * c->store_size);
     if (!e) e = dalloc((void **)&c->boundary, n * 4);
     if (!e) e = dalloc((void **)&c->b1, n * 4);
     if (!e) e = dalloc((void **)&c->b2, n * 4);

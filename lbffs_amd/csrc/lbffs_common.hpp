@@ -116,11 +116,72 @@ __device__ inline T *scalar_base(T *p)
     return (T *)(gT *)a;
 }
 
+// x held as an opaque wave-uniform value in a scalar register: the compiler cannot relate a
+// value computed from it back to the lane's cell index (it would otherwise rewrite the
+// pulls' uniform base plus lane offset into per-lane 64-bit arithmetic).  No instruction.
+__device__ inline uint32_t scalar_u32(uint32_t x)
+{
+    asm("" : "+s"(x));
+    return x;
+}
+
 // work-groups of the face-cell list at the head of a one-launch step (face cells first):
 // rounded up to a multiple of 8, the number of XCDs the hardware deals blocks over in turn
 __host__ __device__ inline uint32_t merged_face_blocks(uint32_t nface, uint32_t block)
 {
     return ((nface + block - 1) / block + 7u) & ~7u;
+}
+
+// ---- population layout on the device ----
+// 32- and 64-bit values: the cells are grouped in blocks of 2^lb consecutive cells, and a
+// block holds its Q directions one after another, each as 2^lb consecutive values: direction
+// q of cell c is element ((c >> lb) * Q + q) * 2^lb + (c mod 2^lb).  A wave of 64 aligned
+// cells stores each direction as one contiguous run, and the Q streams a block reads and
+// writes lie within Q * 2^lb values of each other rather than a whole lattice apart (8192
+// cells for 32-bit values, 4096 for 64-bit: the fastest of the memory skeleton of the step).
+// 16-bit values: Q arrays of the whole lattice, fs values apart (direction-major, fs from
+// population_stride).  Blocks of 64 such values made the skeleton faster but the step
+// slower, whose extra address work the 16-bit forms, bound by instruction issue, feel.
+__host__ __device__ constexpr uint32_t pop_lb(size_t store_size)
+{
+    return store_size == 4 ? 13u : 12u;
+}
+template <int Q, typename S>
+__host__ __device__ inline size_t pop_at(int q, uint32_t c, size_t fs)
+{
+    if constexpr (sizeof(S) == 2) {
+        return (size_t)q * fs + c;
+    } else {
+        constexpr uint32_t lb = pop_lb(sizeof(S));
+        return ((((size_t)(c >> lb)) * (size_t)Q + (size_t)q) << lb) + (c & ((1u << lb) - 1u));
+    }
+}
+
+// Elements between two direction arrays of 16-bit values: the first count >= n whose byte
+// size is 8 KiB past a multiple of 256 KiB.  Arrays whose sizes are large powers of two
+// apart fall into the same cache sets and HBM channels when a cell's 19 or 27 pulls go
+// out together; the 8 KiB skew spreads the directions over 32 slots of a 256 KiB span.
+// (Measured on 256^3 lattices, whose arrays are 2^25 bytes: the 16-bit-storage step
+// ran 27 % slower than on 255 x 256 x 256.)
+inline size_t population_stride(size_t n, size_t elem)
+{
+    const size_t span = size_t(1) << 18, skew = 8192;
+    const size_t bytes = n * elem;
+    const size_t padded = (bytes <= skew) ? skew : ((bytes - skew + span - 1) / span) * span + skew;
+    return padded / elem;
+}
+
+// elements of the population array of N cells: whole blocks, or Q strided arrays
+inline size_t pop_elems(int Q, size_t store_size, size_t N)
+{
+    if (store_size == 2) return (size_t)Q * population_stride(N, store_size);
+    const uint32_t lb = pop_lb(store_size);
+    return (((N + ((size_t)1 << lb) - 1) >> lb) * (size_t)Q) << lb;
+}
+// the stride pop_at takes: the direction arrays' for 16-bit values, unused by blocks
+inline size_t pop_stride(size_t store_size, size_t N)
+{
+    return store_size == 2 ? population_stride(N, store_size) : 0;
 }
 
 // ---- storage codecs ----

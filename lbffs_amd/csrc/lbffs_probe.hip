@@ -4,9 +4,16 @@
 // factor used to turn the step kernel's counters into HBM traffic per launch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "../../include/lbffs.h"
+#include "lbffs_common.hpp"
 
 namespace {
+
+using lbffs::pop_at;
+using lbffs::pop_elems;
+using lbffs::pop_lb;
 
 // mode 0: grid-stride float4 copy
 __global__ void __launch_bounds__(256) lbffs_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
@@ -233,11 +240,28 @@ __global__ void __launch_bounds__(256) probe_lattice_kernel(const T *__restrict_
     for (int q = 0; q < Q; q++) __builtin_nontemporal_store(v[q], dst + (size_t)q * FS + ic);
 }
 
+// the same skeleton on the engine's population blocks (pop_at): cell c of the lattice is
+// cell c + guard of the arrays, guard a whole number of blocks >= the largest pull offset
+template <typename T, int Q>
+__global__ void __launch_bounds__(256) probe_blocks_kernel(const T *__restrict__ src, T *__restrict__ dst,
+                                                           uint32_t guard, int sy, int sz)
+{
+    const uint32_t ic = blockIdx.x * 256u + threadIdx.x + guard;
+    T v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int off = ProbeDirs<Q>::cx[q] + ProbeDirs<Q>::cy[q] * sy + ProbeDirs<Q>::cz[q] * sz;
+        v[q] = src[pop_at<Q, T>(q, (uint32_t)((int)ic - off), 0)];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) __builtin_nontemporal_store(v[q], dst + pop_at<Q, T>(q, ic, 0));
+}
+
 template <typename T, int Q>
 void launch_lattice(int variant, unsigned grid, const T *a, T *b, size_t FS, int sy, int sz)
 {
     switch (variant) {
-    case 0: hipLaunchKernelGGL((probe_lattice_kernel<T, Q, 0>), dim3(grid), dim3(256), 0, 0, a, b, FS, sy, sz); break;
+    case 4: hipLaunchKernelGGL((probe_lattice_kernel<T, Q, 0>), dim3(grid), dim3(256), 0, 0, a, b, FS, sy, sz); break;
     case 1: hipLaunchKernelGGL((probe_lattice_kernel<T, Q, 1>), dim3(grid), dim3(256), 0, 0, a, b, FS, sy, sz); break;
     case 2: hipLaunchKernelGGL((probe_lattice_kernel<T, Q, 2>), dim3(grid), dim3(256), 0, 0, a, b, FS, sy, sz); break;
     default: hipLaunchKernelGGL((probe_lattice_kernel<T, Q, 3>), dim3(grid), dim3(256), 0, 0, a, b, FS, sy, sz); break;
@@ -346,7 +370,7 @@ extern "C" int lbffs_probe_stream(int kind, int unroll, int nt, int block, int g
 extern "C" int lbffs_probe_lattice(int q, int store_bytes, int variant, int nx, int ny, int nz, int reps,
                                    float *ms_per_pass)
 {
-    if ((q != 19 && q != 27) || (store_bytes != 4 && store_bytes != 2) || variant < 0 || variant > 3 || nx < 2 ||
+    if ((q != 19 && q != 27) || (store_bytes != 4 && store_bytes != 2) || variant < 0 || variant > 4 || nx < 2 ||
         ny < 2 || nz < 2 || reps < 1 || !ms_per_pass)
         return LBFFS_E_INVALID;
     const size_t N = (size_t)nx * ny * nz;
@@ -358,7 +382,14 @@ extern "C" int lbffs_probe_lattice(int q, int store_bytes, int variant, int nx, 
     const size_t span = size_t(1) << 18, skew = 8192;
     const size_t raw = (N + 2 * guard) * store_bytes;
     const size_t FS = (((raw - skew + span - 1) / span) * span + skew) / store_bytes;
-    const size_t bytes = ((size_t)q * FS + 2 * guard) * store_bytes;
+    // variant 0, the engine's layout: population blocks for 4-byte values, direction arrays
+    // (variant 4) for 2-byte ones; blocks get a guard of whole blocks on either side
+    const int v = (variant == 0 && store_bytes == 2) ? 4 : variant;
+    const size_t blk = (size_t)1 << pop_lb((size_t)store_bytes);
+    const size_t bguard = (1 + (size_t)sy + (size_t)sz + blk - 1) / blk * blk;
+    if (N + 2 * bguard >= (1ull << 31)) return LBFFS_E_INVALID;
+    const size_t bbytes = pop_elems(q, (size_t)store_bytes, N + 2 * bguard) * store_bytes;
+    const size_t bytes = std::max(((size_t)q * FS + 2 * guard) * store_bytes, bbytes);
     void *a = nullptr, *b = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = LBFFS_OK;
@@ -370,16 +401,24 @@ extern "C" int lbffs_probe_lattice(int q, int store_bytes, int variant, int nx, 
     } else {
         const unsigned grid = (unsigned)(N / 256);
         auto go = [&]() {
-            if (store_bytes == 4) {
+            if (v == 0) {
+                const uint32_t g = (uint32_t)bguard;
+                if (q == 19)
+                    hipLaunchKernelGGL((probe_blocks_kernel<float, 19>), dim3(grid), dim3(256), 0, 0, (const float *)a,
+                                       (float *)b, g, sy, sz);
+                else
+                    hipLaunchKernelGGL((probe_blocks_kernel<float, 27>), dim3(grid), dim3(256), 0, 0, (const float *)a,
+                                       (float *)b, g, sy, sz);
+            } else if (store_bytes == 4) {
                 const float *s = (const float *)a + guard;
                 float *d = (float *)b + guard;
-                if (q == 19) launch_lattice<float, 19>(variant, grid, s, d, FS, sy, sz);
-                else launch_lattice<float, 27>(variant, grid, s, d, FS, sy, sz);
+                if (q == 19) launch_lattice<float, 19>(v, grid, s, d, FS, sy, sz);
+                else launch_lattice<float, 27>(v, grid, s, d, FS, sy, sz);
             } else {
                 const uint16_t *s = (const uint16_t *)a + guard;
                 uint16_t *d = (uint16_t *)b + guard;
-                if (q == 19) launch_lattice<uint16_t, 19>(variant, grid, s, d, FS, sy, sz);
-                else launch_lattice<uint16_t, 27>(variant, grid, s, d, FS, sy, sz);
+                if (q == 19) launch_lattice<uint16_t, 19>(v, grid, s, d, FS, sy, sz);
+                else launch_lattice<uint16_t, 27>(v, grid, s, d, FS, sy, sz);
             }
         };
         go();
