@@ -142,14 +142,19 @@ __host__ __device__ inline uint32_t merged_face_blocks(uint32_t nface, uint32_t 
 // 16-bit values: Q arrays of the whole lattice, fs values apart (direction-major, fs from
 // population_stride).  Blocks of 64 such values made the skeleton faster but the step
 // slower, whose extra address work the 16-bit forms, bound by instruction issue, feel.
+// whether values of this width live in direction arrays (otherwise in population blocks)
+__host__ __device__ constexpr bool pop_arrays(size_t store_size)
+{
+    return store_size == 2;
+}
 __host__ __device__ constexpr uint32_t pop_lb(size_t store_size)
 {
-    return store_size == 4 ? 13u : 12u;
+    return store_size == 8 ? 12u : 13u;
 }
 template <int Q, typename S>
 __host__ __device__ inline size_t pop_at(int q, uint32_t c, size_t fs)
 {
-    if constexpr (sizeof(S) == 2) {
+    if constexpr (pop_arrays(sizeof(S))) {
         return (size_t)q * fs + c;
     } else {
         constexpr uint32_t lb = pop_lb(sizeof(S));
@@ -174,14 +179,14 @@ inline size_t population_stride(size_t n, size_t elem)
 // elements of the population array of N cells: whole blocks, or Q strided arrays
 inline size_t pop_elems(int Q, size_t store_size, size_t N)
 {
-    if (store_size == 2) return (size_t)Q * population_stride(N, store_size);
+    if (pop_arrays(store_size)) return (size_t)Q * population_stride(N, store_size);
     const uint32_t lb = pop_lb(store_size);
     return (((N + ((size_t)1 << lb) - 1) >> lb) * (size_t)Q) << lb;
 }
 // the stride pop_at takes: the direction arrays' for 16-bit values, unused by blocks
 inline size_t pop_stride(size_t store_size, size_t N)
 {
-    return store_size == 2 ? population_stride(N, store_size) : 0;
+    return pop_arrays(store_size) ? population_stride(N, store_size) : 0;
 }
 
 // ---- storage codecs ----

@@ -11,8 +11,7 @@
 
 namespace {
 
-using lbffs::pop_at;
-using lbffs::pop_elems;
+using lbffs::pop_arrays;
 using lbffs::pop_lb;
 
 // mode 0: grid-stride float4 copy
@@ -240,7 +239,16 @@ __global__ void __launch_bounds__(256) probe_lattice_kernel(const T *__restrict_
     for (int q = 0; q < Q; q++) __builtin_nontemporal_store(v[q], dst + (size_t)q * FS + ic);
 }
 
-// the same skeleton on the engine's population blocks (pop_at): cell c of the lattice is
+// direction q of cell c in population blocks of 2^pop_lb cells (the engine's layout for the
+// widths that pop_arrays leaves to blocks)
+template <int Q, typename T>
+__device__ inline size_t blk_at(int q, uint32_t c)
+{
+    constexpr uint32_t lb = pop_lb(sizeof(T));
+    return ((((size_t)(c >> lb)) * (size_t)Q + (size_t)q) << lb) + (c & ((1u << lb) - 1u));
+}
+
+// the same skeleton on population blocks (blk_at): cell c of the lattice is
 // cell c + guard of the arrays, guard a whole number of blocks >= the largest pull offset
 template <typename T, int Q>
 __global__ void __launch_bounds__(256) probe_blocks_kernel(const T *__restrict__ src, T *__restrict__ dst,
@@ -251,10 +259,10 @@ __global__ void __launch_bounds__(256) probe_blocks_kernel(const T *__restrict__
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const int off = ProbeDirs<Q>::cx[q] + ProbeDirs<Q>::cy[q] * sy + ProbeDirs<Q>::cz[q] * sz;
-        v[q] = src[pop_at<Q, T>(q, (uint32_t)((int)ic - off), 0)];
+        v[q] = src[blk_at<Q, T>(q, (uint32_t)((int)ic - off))];
     }
 #pragma unroll
-    for (int q = 0; q < Q; q++) __builtin_nontemporal_store(v[q], dst + pop_at<Q, T>(q, ic, 0));
+    for (int q = 0; q < Q; q++) __builtin_nontemporal_store(v[q], dst + blk_at<Q, T>(q, ic));
 }
 
 template <typename T, int Q>
@@ -384,11 +392,11 @@ extern "C" int lbffs_probe_lattice(int q, int store_bytes, int variant, int nx, 
     const size_t FS = (((raw - skew + span - 1) / span) * span + skew) / store_bytes;
     // variant 0, the engine's layout: population blocks for 4-byte values, direction arrays
     // (variant 4) for 2-byte ones; blocks get a guard of whole blocks on either side
-    const int v = (variant == 0 && store_bytes == 2) ? 4 : variant;
+    const int v = (variant == 0 && pop_arrays((size_t)store_bytes)) ? 4 : variant;
     const size_t blk = (size_t)1 << pop_lb((size_t)store_bytes);
     const size_t bguard = (1 + (size_t)sy + (size_t)sz + blk - 1) / blk * blk;
     if (N + 2 * bguard >= (1ull << 31)) return LBFFS_E_INVALID;
-    const size_t bbytes = pop_elems(q, (size_t)store_bytes, N + 2 * bguard) * store_bytes;
+    const size_t bbytes = (N + 2 * bguard + blk - 1) / blk * blk * (size_t)q * store_bytes;
     const size_t bytes = std::max(((size_t)q * FS + 2 * guard) * store_bytes, bbytes);
     void *a = nullptr, *b = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -403,12 +411,18 @@ extern "C" int lbffs_probe_lattice(int q, int store_bytes, int variant, int nx, 
         auto go = [&]() {
             if (v == 0) {
                 const uint32_t g = (uint32_t)bguard;
-                if (q == 19)
+                if (store_bytes == 4 && q == 19)
                     hipLaunchKernelGGL((probe_blocks_kernel<float, 19>), dim3(grid), dim3(256), 0, 0, (const float *)a,
                                        (float *)b, g, sy, sz);
-                else
+                else if (store_bytes == 4)
                     hipLaunchKernelGGL((probe_blocks_kernel<float, 27>), dim3(grid), dim3(256), 0, 0, (const float *)a,
                                        (float *)b, g, sy, sz);
+                else if (q == 19)
+                    hipLaunchKernelGGL((probe_blocks_kernel<uint16_t, 19>), dim3(grid), dim3(256), 0, 0,
+                                       (const uint16_t *)a, (uint16_t *)b, g, sy, sz);
+                else
+                    hipLaunchKernelGGL((probe_blocks_kernel<uint16_t, 27>), dim3(grid), dim3(256), 0, 0,
+                                       (const uint16_t *)a, (uint16_t *)b, g, sy, sz);
             } else if (store_bytes == 4) {
                 const float *s = (const float *)a + guard;
                 float *d = (float *)b + guard;
