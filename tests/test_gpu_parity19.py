@@ -220,6 +220,26 @@ def test_f64_vs_oracle_f64():
     compare(h, o, keys=("rho", "vel", "f", "tauSGS"), tol=1e-11, label="f64")
 
 
+@pytest.mark.parametrize("storage", [lb.STORE_F32, lb.STORE_F64])
+def test_population_blocks_partial_block_and_per_lane_paths(storage):
+    # 70 x 32 x 9 = 20160 cells: the population blocks (8192 cells FP32, 4096 FP64) end in a
+    # partial one, and rows of 70 cells are not whole waves.  In the y-tiled order (launch
+    # option 4, tiles of 16 rows) no wave is an aligned run, so every pull and store
+    # addresses its cell per lane; in the plain order they take the wave's block base.  The
+    # inlet, outlet, walls, periodic z and the solid cylinder take the face, bounce-back and
+    # wrap paths.  Both orders give the same bits, and they match the oracle.
+    c = cases.cylinder_channel(nx=70, ny=32, nz=9, D=8, cx=20, cy=16, spz_cells=6)
+    plain = run_hip(c, 40, storage=storage)
+    lanes = run_hip(c, 40, storage=storage, tune=4)
+    for k in ("f", "rho", "u", "v", "w", "tauSGS", "Fwx", "Fwy", "Fwz"):
+        assert np.array_equal(plain[k], lanes[k]), k
+    if storage == lb.STORE_F32:
+        parity(c, 40, label="blocks 70x32x9", h=plain)
+    else:
+        o = po.run19(c, 40, order=po.JACOBI, precision="f64")
+        compare(plain, o, keys=("rho", "vel", "f", "tauSGS"), tol=1e-11, label="blocks f64")
+
+
 def test_reference_kernel_cavity_short():
     """The reference kernel.cl on the GPU against the oracle and the HIP step."""
     c = cases.cavity(32)
