@@ -218,9 +218,10 @@ enum {
 };
 
 /* Autotuner: time every candidate work-group size (64, 128, 256) and launch option on
- * this lattice for trial_steps steps each, keep the fastest for later lbffs_step calls
- * and report it.  Populations and every field a step writes are restored afterwards, so
- * the simulation state is unchanged. */
+ * this lattice for trial_steps steps each, time the three fastest again over 2 trial_steps
+ * in two interleaved rounds, keep the best of those for later lbffs_step calls and report
+ * it (best_ms from the second stage).  Populations and every field a step writes are
+ * restored afterwards, so the simulation state is unchanged. */
 int lbffs_autotune(lbffs_ctx *ctx, const lbffs_step_params *params, int trial_steps, int *best_block,
                    int *best_tune, float *best_ms_per_step);
 

@@ -1409,30 +1409,53 @@ int lbffs_autotune(lbffs_ctx *c, const lbffs_step_params *p, int trial_steps, in
         tunes[3] = TUNE_YTILE;
         ntunes = ytile_rows(c) ? 4 : 3;
     }
+    // device time per step of one shape over n steps, after one warming step
+    auto time_shape = [&](int block, int tune, int n, float *ms) {
+        c->block = block;
+        c->tune = tune;
+        rc = dispatch<StepOp>(c, 1, p);
+        if (!rc) he = hipEventRecord(c->ev0, c->stream);
+        if (!rc && he == hipSuccess) rc = dispatch<StepOp>(c, n, p);
+        if (!rc && he == hipSuccess) he = hipEventRecord(c->ev1, c->stream);
+        if (!rc && he == hipSuccess) he = hipEventSynchronize(c->ev1);
+        *ms = 0.f;
+        if (!rc && he == hipSuccess) he = hipEventElapsedTime(ms, c->ev0, c->ev1);
+        *ms /= n;
+        return !rc && he == hipSuccess;
+    };
+    // every shape once; then the three fastest again over twice the steps, in two interleaved
+    // rounds, keeping each one's best: shapes within a few per cent of each other are told
+    // apart by more than one short trial
+    struct Shape {
+        float ms;
+        int block, tune;
+    };
+    std::vector<Shape> shapes;
+    for (int bi = 0; bi < 3; bi++) {
+        for (int ti = 0; ti < ntunes; ti++) {
+            float ms;
+            if (!time_shape(blocks[bi], tunes[ti], trial_steps, &ms)) break;
+            shapes.push_back({ms, blocks[bi], tunes[ti]});
+        }
+        if (rc || he != hipSuccess) break;
+    }
+    std::sort(shapes.begin(), shapes.end(), [](const Shape &x, const Shape &y) { return x.ms < y.ms; });
+    const size_t nfinal = std::min<size_t>(3, shapes.size());
+    std::vector<float> best(nfinal, 1e30f);
+    for (int round = 0; round < 2 && rc == 0 && he == hipSuccess; round++)
+        for (size_t k = 0; k < nfinal; k++) {
+            float ms;
+            if (!time_shape(shapes[k].block, shapes[k].tune, 2 * trial_steps, &ms)) break;
+            best[k] = std::min(best[k], ms);
+        }
     float bms = 1e30f;
     int bb = block0, bt = tune0;
-    for (int bi = 0; bi < 3 && he == hipSuccess && rc == 0; bi++) {
-        for (int ti = 0; ti < ntunes && he == hipSuccess && rc == 0; ti++) {
-            c->block = blocks[bi];
-            c->tune = tunes[ti];
-            rc = dispatch<StepOp>(c, 1, p);  // warm the shape
-            if (rc) break;
-            he = hipEventRecord(c->ev0, c->stream);
-            if (he != hipSuccess) break;
-            rc = dispatch<StepOp>(c, trial_steps, p);
-            if (rc) break;
-            he = hipEventRecord(c->ev1, c->stream);
-            if (he == hipSuccess) he = hipEventSynchronize(c->ev1);
-            float ms = 0.f;
-            if (he == hipSuccess) he = hipEventElapsedTime(&ms, c->ev0, c->ev1);
-            ms /= trial_steps;
-            if (he == hipSuccess && ms < bms) {
-                bms = ms;
-                bb = blocks[bi];
-                bt = tunes[ti];
-            }
+    for (size_t k = 0; k < nfinal; k++)
+        if (best[k] < bms) {
+            bms = best[k];
+            bb = shapes[k].block;
+            bt = shapes[k].tune;
         }
-    }
     // restore the state
     const hipError_t hr = saved.restore();
     if (he == hipSuccess) he = hr;
