@@ -1,8 +1,9 @@
 // d3q19_step.hpp -- the fused D3Q19 time-step kernel.
 //
 // One work-item per cell, cells flattened as ic = i + nx*(j + ny*k) so that a
-// wavefront's 64 lanes touch 64 consecutive cells of one population array: every
-// pull load and every store is a coalesced run of 256 B (FP32) or 128 B (16-bit).
+// wavefront's 64 lanes touch 64 consecutive cells of one direction (in a population block
+// or a direction array, pop_at): every store is a coalesced run of 256 B (FP32) or 128 B
+// (16-bit), every pull one or two.
 // The kernel does, for each fluid cell, what k_streamingCollision does
 // (D3Q19_opencl/kernel.cl:3547-3645):
 //   pull streaming (interior: fixed offsets; faces: wrap or wall per face code),

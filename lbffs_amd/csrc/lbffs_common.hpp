@@ -102,7 +102,7 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f)
 }
 
 // p held as an opaque wave-uniform value in scalar registers.  Stores of one lane offset
-// into several arrays (new populations: direction q at fd + q FS + offset) would otherwise
+// from several bases (new populations: direction q at its own base + offset) would otherwise
 // be rewritten by the compiler into one 64-bit vector address advanced by a 64-bit vector
 // add per store; through this barrier each store takes its array's scalar base and the
 // shared 32-bit lane offset.  (An empty asm statement: no instruction is emitted.)
