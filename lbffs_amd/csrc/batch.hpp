@@ -62,8 +62,15 @@ __host__ inline unsigned batch_grid(unsigned grid, uint32_t n)
     return g >= n ? n : g;
 }
 
+// Waves per SIMD the D3Q19 block-table pass is held to: eight with WALE (64 VGPRs, no spill;
+// the ensemble's step 6.63 -> 6.22 ms), seven without a sub-grid model, where eight loads
+// fewer populations in flight and ran slower (5.30 -> 5.92 ms).
+template <int SGS, bool IBB, int PASS, typename Real>
+constexpr int kBatch19MinWaves = (IBB || PASS != 1 || sizeof(Real) == 8) ? 1 : (SGS == 0 ? 8 : 7);
+
 template <typename S, typename Real, int SGS, int Q, bool IBB, int PASS, int COLL = COLL_CUMULANT>
-__global__ void __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(Q == 27 ? kBatch27MinWaves<Real> : (IBB || PASS != 1 || sizeof(Real) == 8 ? 1 : 7), 8)))
+__global__ void __launch_bounds__(kBatchBlock)
+__attribute__((amdgpu_waves_per_eu(Q == 27 ? kBatch27MinWaves<Real> : kBatch19MinWaves<SGS, IBB, PASS, Real>, 8)))
     batch_step_kernel(const StepArgs<Real, Q> *__restrict__ args, const BatchBlock *__restrict__ table,
                       uint32_t nblocks, const uint32_t *__restrict__ cells)
 {
