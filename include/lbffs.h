@@ -322,7 +322,8 @@ int lbffs_probe_stream(int kind, int unroll, int nt, int block, int grid, size_t
  * straddle cache lines); 1 the x component of every offset dropped (aligned runs: the ideal
  * pull); 2 x-shifted directions loaded aligned and moved one lane by a DPP wave shift, the
  * lane leaving the wave loading its value alone; 3 as 2, with that lane's value passed
- * between the four waves of a work-group through LDS. */
+ * between the four waves of a work-group through LDS.  100 + lb (lb = 6..18): population
+ * blocks of 2^lb cells, whatever the width (the sweep behind the engine's block sizes). */
 int lbffs_probe_lattice(int q, int store_bytes, int variant, int nx, int ny, int nz, int reps, float *ms_per_pass);
 
 #ifdef __cplusplus

@@ -265,6 +265,25 @@ __global__ void __launch_bounds__(256) probe_blocks_kernel(const T *__restrict__
     for (int q = 0; q < Q; q++) __builtin_nontemporal_store(v[q], dst + blk_at<Q, T>(q, ic));
 }
 
+// the same skeleton on population blocks of 2^lb cells, lb chosen at run time (the sweep
+// behind the engine's block sizes)
+template <typename T, int Q>
+__global__ void __launch_bounds__(256) probe_blocks_rt_kernel(const T *__restrict__ src, T *__restrict__ dst,
+                                                              uint32_t guard, int sy, int sz, uint32_t lb)
+{
+    const uint32_t ic = blockIdx.x * 256u + threadIdx.x + guard;
+    const uint32_t m = (1u << lb) - 1u;
+    T v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const uint32_t c = (uint32_t)((int)ic - (ProbeDirs<Q>::cx[q] + ProbeDirs<Q>::cy[q] * sy + ProbeDirs<Q>::cz[q] * sz));
+        v[q] = src[((((size_t)(c >> lb)) * Q + q) << lb) + (c & m)];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        __builtin_nontemporal_store(v[q], dst + ((((size_t)(ic >> lb)) * Q + q) << lb) + (ic & m));
+}
+
 template <typename T, int Q>
 void launch_lattice(int variant, unsigned grid, const T *a, T *b, size_t FS, int sy, int sz)
 {
@@ -378,7 +397,8 @@ extern "C" int lbffs_probe_stream(int kind, int unroll, int nt, int block, int g
 extern "C" int lbffs_probe_lattice(int q, int store_bytes, int variant, int nx, int ny, int nz, int reps,
                                    float *ms_per_pass)
 {
-    if ((q != 19 && q != 27) || (store_bytes != 4 && store_bytes != 2) || variant < 0 || variant > 4 || nx < 2 ||
+    const bool rt = variant >= 106 && variant <= 118;  // blocks of 2^(variant - 100) cells
+    if ((q != 19 && q != 27) || (store_bytes != 4 && store_bytes != 2) || variant < 0 || (variant > 4 && !rt) || nx < 2 ||
         ny < 2 || nz < 2 || reps < 1 || !ms_per_pass)
         return LBFFS_E_INVALID;
     const size_t N = (size_t)nx * ny * nz;
@@ -393,7 +413,8 @@ extern "C" int lbffs_probe_lattice(int q, int store_bytes, int variant, int nx, 
     // variant 0, the engine's layout: population blocks for 4-byte values, direction arrays
     // (variant 4) for 2-byte ones; blocks get a guard of whole blocks on either side
     const int v = (variant == 0 && pop_arrays((size_t)store_bytes)) ? 4 : variant;
-    const size_t blk = (size_t)1 << pop_lb((size_t)store_bytes);
+    const uint32_t lb = rt ? (uint32_t)(variant - 100) : pop_lb((size_t)store_bytes);
+    const size_t blk = (size_t)1 << lb;
     const size_t bguard = (1 + (size_t)sy + (size_t)sz + blk - 1) / blk * blk;
     if (N + 2 * bguard >= (1ull << 31)) return LBFFS_E_INVALID;
     const size_t bbytes = (N + 2 * bguard + blk - 1) / blk * blk * (size_t)q * store_bytes;
@@ -409,7 +430,21 @@ extern "C" int lbffs_probe_lattice(int q, int store_bytes, int variant, int nx, 
     } else {
         const unsigned grid = (unsigned)(N / 256);
         auto go = [&]() {
-            if (v == 0) {
+            if (rt) {
+                const uint32_t g = (uint32_t)bguard;
+                if (store_bytes == 4 && q == 19)
+                    hipLaunchKernelGGL((probe_blocks_rt_kernel<float, 19>), dim3(grid), dim3(256), 0, 0,
+                                       (const float *)a, (float *)b, g, sy, sz, lb);
+                else if (store_bytes == 4)
+                    hipLaunchKernelGGL((probe_blocks_rt_kernel<float, 27>), dim3(grid), dim3(256), 0, 0,
+                                       (const float *)a, (float *)b, g, sy, sz, lb);
+                else if (q == 19)
+                    hipLaunchKernelGGL((probe_blocks_rt_kernel<uint16_t, 19>), dim3(grid), dim3(256), 0, 0,
+                                       (const uint16_t *)a, (uint16_t *)b, g, sy, sz, lb);
+                else
+                    hipLaunchKernelGGL((probe_blocks_rt_kernel<uint16_t, 27>), dim3(grid), dim3(256), 0, 0,
+                                       (const uint16_t *)a, (uint16_t *)b, g, sy, sz, lb);
+            } else if (v == 0) {
                 const uint32_t g = (uint32_t)bguard;
                 if (store_bytes == 4 && q == 19)
                     hipLaunchKernelGGL((probe_blocks_kernel<float, 19>), dim3(grid), dim3(256), 0, 0, (const float *)a,
