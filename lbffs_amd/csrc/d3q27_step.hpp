@@ -624,9 +624,11 @@ __device__ inline void collide_rr27(Real *ft, Real *K, const Mom19<Real> &m, Rea
 
 // At least seven waves per SIMD (72 VGPRs) for the 32-bit arithmetic: the cumulant update
 // is VALU bound and fits without spilling (eight waves, 64 VGPRs, would spill); the
-// compiler's own choice went as low as five.  The FP64 form keeps the compiler's choice.
-template <typename Real>
-constexpr int kStep27MinWaves = sizeof(Real) == 8 ? 1 : 7;
+// compiler's own choice went as low as five.  FP32 populations with WALE: six (76-78 VGPRs),
+// as the population blocks' pull spills 8 B at seven and the spill-free six ran the bluff
+// body's step in 5.82 against 6.03 ms.  The FP64 form keeps the compiler's choice.
+template <typename S, typename Real, int SGS>
+constexpr int kStep27MinWaves = sizeof(Real) == 8 ? 1 : ((sizeof(S) == 4 && SGS == 0) ? 6 : 7);
 // the batch kernel's extra indirection spills at seven: six
 template <typename Real>
 constexpr int kBatch27MinWaves = sizeof(Real) == 8 ? 1 : 6;
@@ -634,7 +636,7 @@ constexpr int kBatch27MinWaves = sizeof(Real) == 8 ? 1 : 6;
 // PASS as step19_kernel: 1 the cells off the lattice faces, 2 the listed face cells, 3 both
 // in one grid (the face-list work-groups first)
 template <typename S, typename Real, int SGS, int PASS = 0, int COLL = COLL_CUMULANT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStep27MinWaves<Real>, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStep27MinWaves<S, Real, SGS>, 8)))
 step27_kernel(const StepArgs<Real, 27> a)
 {
     const uint32_t nfb = (PASS == 3) ? face_blocks(a) : 0u;
