@@ -420,7 +420,7 @@ int dalloc(void **p, size_t bytes)
 }
 
 // The class of the run of 64 cells from `first` (wave_class): PURE if all are fluid and off
-// the lattice faces; on D3Q19 also XMIX if all are fluid and the face cells among them lie
+// the lattice faces (PLAIN if besides every flag word is zero); on D3Q19 also XMIX if all are fluid and the face cells among them lie
 // on an x face with j and k interior (so that every cell's interior-offset pulls stay in the
 // lattice), or WRAP if all are fluid cells of one row with i interior, on a y face and/or a z
 // face whose code is Cyclic for every cell (their pulls wrap like interior pulls shifted by
@@ -428,12 +428,13 @@ int dalloc(void **p, size_t bytes)
 uint32_t wave_class_of(const lbffs_ctx *c, const uint16_t *fl, size_t first)
 {
     const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;
-    bool face = false;
+    bool face = false, plain = true;
     for (size_t l = 0; l < 64; l++) {
         if (fl[first + l] & F_SOLID) return WAVE_GENERAL;
         face = face || (fl[first + l] & F_FACE);
+        plain = plain && fl[first + l] == 0;
     }
-    if (!face) return WAVE_PURE;
+    if (!face) return plain ? WAVE_PLAIN : WAVE_PURE;
     if (c->d.q != 19) return WAVE_GENERAL;
     bool xmix = nz >= 3, wrap = true;
     const size_t i0 = first % (size_t)nx, r0 = first / (size_t)nx;
@@ -516,12 +517,12 @@ int pack_flags(lbffs_ctx *c)
             HIP_TRY(hipMemcpy(c->fcells, cells.data(), cells.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
             c->nface = (uint32_t)cells.size();
         }
-        // per-wave summary (wave_class), 2 bits per run of 64 cells
+        // per-wave summary (wave_class), 4 bits per run of 64 cells
         const size_t nw = (size_t)c->N / 64;
-        std::vector<uint32_t> bits((nw + 15) / 16 + 1, 0u);
+        std::vector<uint32_t> bits((nw + 7) / 8 + 1, 0u);
         for (size_t w = 0; w < nw; w++) {
             const uint32_t cls = wave_class_of(c, fl.data(), w * 64);
-            bits[w >> 4] |= cls << (2 * (w & 15));
+            bits[w >> 3] |= cls << (4 * (w & 7));
         }
         if (!c->wclass) {
             if (int e = dalloc((void **)&c->wclass, bits.size() * sizeof(uint32_t))) return e;

@@ -32,8 +32,8 @@ enum : uint32_t {
 
 __host__ __device__ inline int flag_code(uint32_t fl, int shift) { return (int)((fl >> shift) & 7u); }
 
-// Per-wave summary of the flags: 2 bits per aligned run of 64 cells (bits 2 (w % 16) of word
-// w / 16 for the run 64w .. 64w + 63), built with the flags:
+// Per-wave summary of the flags: 4 bits per aligned run of 64 cells (bits 4 (w % 8) of word
+// w / 8 for the run 64w .. 64w + 63), built with the flags:
 //   WAVE_GENERAL  every cell reads its flags first and takes its own path;
 //   WAVE_PURE     64 fluid cells off the lattice faces: fixed interior offsets;
 //   WAVE_XMIX     64 fluid cells whose face cells lie on an x face only (their j and k
@@ -41,26 +41,28 @@ __host__ __device__ inline int flag_code(uint32_t fl, int shift) { return (int)(
 //                 face cells redo the directions that cross their face and apply its conditions;
 //   WAVE_WRAP     64 fluid cells of one row, none on an x face, inside a y and/or z face plane
 //                 whose code is Cyclic for all of them: interior offsets with a periodic wrap
-//                 uniform over the wave.
+//                 uniform over the wave;
+//   WAVE_PLAIN    a PURE run whose 64 flag words are all zero (no solid link, no field to keep
+//                 or read): the wave skips its flag load.
 // Such waves issue their pulls together with the flag load instead of one memory round trip
 // after it, and no lane runs the general face path.  Read through the scalar cache (the
 // wave's first active cell names the run; the caller launches 64-cell aligned waves).
-enum : uint32_t { WAVE_GENERAL = 0, WAVE_PURE = 1, WAVE_XMIX = 2, WAVE_WRAP = 3 };
+enum : uint32_t { WAVE_GENERAL = 0, WAVE_PURE = 1, WAVE_XMIX = 2, WAVE_WRAP = 3, WAVE_PLAIN = 4 };
 
 __device__ inline uint32_t wave_class(const uint32_t *bits, uint32_t icu)
 {
     if (!bits) return WAVE_GENERAL;
     const uint32_t w = __builtin_amdgcn_readfirstlane(icu) >> 6;
     typedef const __attribute__((address_space(4))) uint32_t const_u32;
-    const uint32_t word = *((const_u32 *)bits + (w >> 4));
-    return (word >> (2u * (w & 15u))) & 3u;
+    const uint32_t word = *((const_u32 *)bits + (w >> 3));
+    return (word >> (4u * (w & 7u))) & 15u;
 }
 
 // the class of the run holding cell ic, per lane (a vector load)
 __device__ inline uint32_t cell_wave_class(const uint32_t *bits, uint32_t ic)
 {
     if (!bits) return WAVE_GENERAL;
-    return (bits[ic >> 10] >> (2u * ((ic >> 6) & 15u))) & 3u;
+    return (bits[ic >> 9] >> (4u * ((ic >> 6) & 7u))) & 15u;
 }
 
 // BC codes (input.hpp:221-256)
