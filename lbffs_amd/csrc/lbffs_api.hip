@@ -420,11 +420,12 @@ int dalloc(void **p, size_t bytes)
 }
 
 // The class of the run of 64 cells from `first` (wave_class): PURE if all are fluid and off
-// the lattice faces (PLAIN if besides every flag word is zero); on D3Q19 also XMIX if all are fluid and the face cells among them lie
-// on an x face with j and k interior (so that every cell's interior-offset pulls stay in the
-// lattice), or WRAP if all are fluid cells of one row with i interior, on a y face and/or a z
-// face whose code is Cyclic for every cell (their pulls wrap like interior pulls shifted by
-// whole rows or planes, and no face condition applies); GENERAL otherwise.
+// the lattice faces (PLAIN if besides every flag word is zero); on D3Q19 also XMIX if all are
+// fluid and the face cells among them lie on an x face with j and k interior (so that every
+// cell's interior-offset pulls stay in the lattice), or WRAP if all are fluid cells of one row
+// with i interior, on a y face and/or a z face whose code is Cyclic for every cell (their
+// pulls wrap like interior pulls shifted by whole rows or planes, and no face condition
+// applies); GENERAL otherwise.
 uint32_t wave_class_of(const lbffs_ctx *c, const uint16_t *fl, size_t first)
 {
     const int nx = c->d.nx, ny = c->d.ny, nz = c->d.nz;

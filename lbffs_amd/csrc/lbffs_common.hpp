@@ -142,8 +142,9 @@ __host__ __device__ inline uint32_t merged_face_blocks(uint32_t nface, uint32_t 
 // writes lie within Q * 2^lb values of each other rather than a whole lattice apart (8192
 // cells for 32-bit values, 4096 for 64-bit: the fastest of the memory skeleton of the step).
 // 16-bit values: Q arrays of the whole lattice, fs values apart (direction-major, fs from
-// population_stride).  Blocks of 64 such values made the skeleton faster but the step
-// slower, whose extra address work the 16-bit forms, bound by instruction issue, feel.
+// population_stride).  Blocks of 64 or 8192 such values made the skeleton faster but the
+// 16-bit step slower, in every form of the block pull measured.
+
 // whether values of this width live in direction arrays (otherwise in population blocks)
 __host__ __device__ constexpr bool pop_arrays(size_t store_size)
 {
