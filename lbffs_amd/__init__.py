@@ -143,7 +143,8 @@ def lib():
                   "lbffs_autotune", "lbffs_set_launch",
                   "lbffs_ibm_set_markers", "lbffs_ibm_set_motion", "lbffs_ibm_download", "lbffs_batch_create",
                   "lbffs_batch_context", "lbffs_batch_set_launch", "lbffs_batch_step", "lbffs_batch_last_step_ms",
-                  "lbffs_batch_autotune", "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf", "lbffs_geom_qlinks", "lbffs_geom_qlinks_ctx",
+                  "lbffs_batch_autotune", "lbffs_batch_destroy", "lbffs_geom_solid", "lbffs_geom_sdf",
+                  "lbffs_geom_qlinks", "lbffs_geom_qlinks_ctx",
                   "lbffs_case_sdf_near_bwalls",
                   "lbffs_output_macros", "lbffs_out_write_vtk", "lbffs_out_write_restart", "lbffs_out_read_restart",
                   "lbffs_out_stats_compute", "lbffs_out_coeffs_fp64", "lbffs_out_accumulate_mean"):

@@ -825,7 +825,8 @@ template <typename S, typename Real, int SGS, bool IBB>
 constexpr int kStep19MinWaves = sizeof(Real) == 8 ? 1 : ((IBB && SGS == 0 && sizeof(S) == 4) ? 7 : 8);
 
 template <typename S, typename Real, int SGS, bool IBB = false, int AA = 0, int PASS = 0>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStep19MinWaves<S, Real, SGS, IBB>, 8))) step19_kernel(const Step19Args<Real> a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStep19MinWaves<S, Real, SGS, IBB>, 8)))
+step19_kernel(const Step19Args<Real> a)
 {
     const uint32_t nfb = (PASS == 3) ? face_blocks(a) : 0u;
     const bool face_wg = (PASS == 2) || (PASS == 3 && blockIdx.x < nfb);
